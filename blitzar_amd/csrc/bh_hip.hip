@@ -1,0 +1,794 @@
+// C ABI of octree_hip (include/bh_hip.h): handle lifecycle, buffer
+// management and the per-step orchestration of the HIP kernels.
+//
+// Step structure (leapfrog), after the first force evaluation:
+//   kick_drift(dt/2, dt) -> AABB -> keys -> radix sort -> tie fix -> permute
+//   -> tree levels -> moments -> walk -> kick(dt/2)
+// All of it is queued on the handle's stream without host synchronisation;
+// the host synchronises once per API call to report device-side errors.
+#include "../../include/bh_hip.h"
+#include "bh_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+
+constexpr int kDefaultWavesPerCu = 16;
+constexpr size_t kStackCap = 16384;
+
+struct StepEvents {
+    hipEvent_t b0 = nullptr, s0 = nullptr, s1 = nullptr, b1 = nullptr, w1 = nullptr;
+};
+
+}  // namespace
+
+struct bh_hip_handle {
+    bh::Device d;
+    bh_hip_params_t p{};
+    int leaf_depth = 1;
+    int cu_count = 256;
+    bool primed = false;      // acc holds a(x) of the current positions
+    bool have_tree = false;   // the last build matches the current positions
+    bool timing = false;
+    std::string last_error;
+    unsigned long long* stats = nullptr;  // device: pp, pc
+    std::vector<StepEvents> events;
+    size_t events_used = 0;
+    double integrate_ms = 0.0;
+    hipEvent_t i0 = nullptr, i1 = nullptr;
+    bh_hip_timings_t t{};
+    std::vector<void*> allocations;
+};
+
+namespace {
+
+void set_err(char* err, size_t cap, const std::string& msg)
+{
+    if (err && cap > 0) {
+        std::snprintf(err, cap, "%s", msg.c_str());
+    }
+}
+
+bool check(bh_hip_handle* h, hipError_t e, const char* what)
+{
+    if (e == hipSuccess)
+        return true;
+    h->last_error = std::string("[hip] ") + what + " failed: " + hipGetErrorString(e);
+    return false;
+}
+
+template <typename T>
+bool dalloc(bh_hip_handle* h, T** p, size_t count, const char* what)
+{
+    void* q = nullptr;
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (!check(h, hipMalloc(&q, bytes), what))
+        return false;
+    h->allocations.push_back(q);
+    *p = static_cast<T*>(q);
+    return true;
+}
+
+void dfree(bh_hip_handle* h, void* p)
+{
+    if (!p)
+        return;
+    auto it = std::find(h->allocations.begin(), h->allocations.end(), p);
+    if (it != h->allocations.end())
+        h->allocations.erase(it);
+    (void)hipFree(p);
+}
+
+// PhyForceLawPolicy.cpp:22-32 and the octree setters (OctConfiguration.cpp:26-39).
+bh_hip_params_t resolve(const bh_hip_params_t& in)
+{
+    bh_hip_params_t p = in;
+    const float min_soft = std::max(0.0f, in.min_softening);
+    const float min_theta = std::clamp(in.min_theta, 0.05f, 4.0f);
+    p.min_softening = min_soft;
+    p.min_distance2 = std::max(0.0f, in.min_distance2);
+    p.min_theta = min_theta;
+    p.softening = std::max(in.softening, min_soft);
+    p.theta = std::clamp(in.theta, min_theta, 4.0f);
+    p.bucket_capacity = std::max(1, in.bucket_capacity);
+    p.multipole_order = in.multipole_order >= 2 ? 2 : 1;
+    p.criterion = in.criterion == 1 ? 1 : 0;
+    p.key_mode = in.key_mode == 1 ? 1 : 0;
+    p.integrator = in.integrator == 1 ? 1 : 0;
+    if (p.waves_per_cu <= 0)
+        p.waves_per_cu = kDefaultWavesPerCu;
+    return p;
+}
+
+// OctBuild.inl:28-39
+int reference_leaf_depth(uint64_t n, int leaf_capacity)
+{
+    const int cap = std::max(16, leaf_capacity);
+    int d = 1;
+    while (d < 21) {
+        const double avg = static_cast<double>(n) / static_cast<double>(1ull << (3 * d));
+        if (avg <= static_cast<double>(cap))
+            break;
+        ++d;
+    }
+    return d;
+}
+
+bool alloc_nodes(bh_hip_handle* h, size_t cap)
+{
+    bh::Device& d = h->d;
+    dfree(h, d.nprefix);
+    dfree(h, d.nstart);
+    dfree(h, d.ncount);
+    dfree(h, d.nchild);
+    dfree(h, d.nfirst);
+    dfree(h, d.nlevel);
+    dfree(h, d.wnode);
+    dfree(h, d.scan_tmp);
+    dfree(h, d.scan_blocks);
+    d.nprefix = nullptr;
+    d.node_cap = cap;
+    const size_t scan_n = std::min(d.n, cap);
+    return dalloc(h, &d.nprefix, cap, "node prefix") && dalloc(h, &d.nstart, cap, "node start") &&
+           dalloc(h, &d.ncount, cap, "node count") && dalloc(h, &d.nchild, cap, "node nchild") &&
+           dalloc(h, &d.nfirst, cap, "node first") && dalloc(h, &d.nlevel, cap, "node level") &&
+           dalloc(h, &d.wnode, cap, "walk nodes") && dalloc(h, &d.scan_tmp, scan_n, "scan tmp") &&
+           dalloc(h, &d.scan_blocks, bh::scan_block_sums_words(scan_n), "scan blocks");
+}
+
+bool alloc_all(bh_hip_handle* h, uint64_t n)
+{
+    bh::Device& d = h->d;
+    d.n = n;
+    for (int a = 0; a < 3; ++a) {
+        if (!dalloc(h, &d.pos[a], n, "pos") || !dalloc(h, &d.vel[a], n, "vel") ||
+            !dalloc(h, &d.pos_b[a], n, "pos_b") || !dalloc(h, &d.vel_b[a], n, "vel_b") ||
+            !dalloc(h, &d.acc[a], n, "acc"))
+            return false;
+    }
+    if (!dalloc(h, &d.mass, n, "mass") || !dalloc(h, &d.mass_b, n, "mass_b") ||
+        !dalloc(h, &d.id, n, "id") || !dalloc(h, &d.id_b, n, "id_b") ||
+        !dalloc(h, &d.rank, n, "rank") || !dalloc(h, &d.keys, n, "keys") ||
+        !dalloc(h, &d.keys_b, n, "keys_b") || !dalloc(h, &d.vals, n, "vals") ||
+        !dalloc(h, &d.vals_b, n, "vals_b") || !dalloc(h, &d.spart, n, "spart") ||
+        !dalloc(h, &d.sleaf, n, "sleaf") || !dalloc(h, &d.stage, 3 * n, "stage"))
+        return false;
+    d.sort_hist_cap = bh::radix_hist_words(n);
+    if (!dalloc(h, &d.sort_hist, d.sort_hist_cap, "sort hist"))
+        return false;
+    if (!dalloc(h, &d.level_info, 2 * bh::kLevels + 8, "level info") ||
+        !dalloc(h, &d.geom, 1, "geometry") || !dalloc(h, &d.aabb_bits, 8, "aabb") ||
+        !dalloc(h, &d.err, 4, "error flags") || !dalloc(h, &d.work_counter, 4, "work counter") ||
+        !dalloc(h, &h->stats, 2, "walk stats"))
+        return false;
+    d.diag_cap = 8 * 256 + 16;
+    if (!dalloc(h, &d.diag, d.diag_cap, "diagnostics"))
+        return false;
+    if (!alloc_nodes(h, 2 * n + 4096))
+        return false;
+    const size_t groups = (n + 63) / 64;
+    d.stack_waves = std::max<size_t>(
+        1, std::min(groups, static_cast<size_t>(h->cu_count) * static_cast<size_t>(h->p.waves_per_cu)));
+    d.stack_cap = std::min(kStackCap, d.node_cap + 64);
+    return dalloc(h, &d.stack, d.stack_waves * d.stack_cap, "walk stacks");
+}
+
+bool zero_err(bh_hip_handle* h)
+{
+    bh::launch_fill_u32(reinterpret_cast<uint32_t*>(h->d.err), 0u, 1, h->d.stream);
+    return check(h, hipGetLastError(), "error flag reset");
+}
+
+// Synchronise and turn device error bits into a message.
+bh_hip_result_t finish(bh_hip_handle* h, const char* what)
+{
+    if (!check(h, hipGetLastError(), what) || !check(h, hipStreamSynchronize(h->d.stream), what))
+        return BH_HIP_INTERNAL_ERROR;
+    int flags = 0;
+    if (!check(h, hipMemcpy(&flags, h->d.err, sizeof(int), hipMemcpyDeviceToHost), "error read"))
+        return BH_HIP_INTERNAL_ERROR;
+    if (flags != 0) {
+        std::string m = std::string("[octree_hip] ") + what + ":";
+        if (flags & bh::kErrNodeOverflow)
+            m += " node capacity exceeded (capacity grown; results of this call degraded);";
+        if (flags & bh::kErrStackOverflow)
+            m += " traversal stack overflow;";
+        if (flags & bh::kErrLeafTooLarge)
+            m += " leaf larger than 2^23 particles;";
+        h->last_error = m;
+        if (flags & bh::kErrNodeOverflow)
+            alloc_nodes(h, h->d.node_cap * 2);
+        return BH_HIP_INTERNAL_ERROR;
+    }
+    return BH_HIP_OK;
+}
+
+StepEvents* next_events(bh_hip_handle* h)
+{
+    if (!h->timing)
+        return nullptr;
+    if (h->events_used == h->events.size()) {
+        StepEvents e;
+        if (hipEventCreate(&e.b0) != hipSuccess || hipEventCreate(&e.s0) != hipSuccess ||
+            hipEventCreate(&e.s1) != hipSuccess || hipEventCreate(&e.b1) != hipSuccess ||
+            hipEventCreate(&e.w1) != hipSuccess)
+            return nullptr;
+        h->events.push_back(e);
+    }
+    return &h->events[h->events_used++];
+}
+
+void rec(hipEvent_t e, hipStream_t s)
+{
+    if (e)
+        (void)hipEventRecord(e, s);
+}
+
+// Queue one tree build for the current positions.
+void queue_build(bh_hip_handle* h, StepEvents* ev)
+{
+    bh::Device& d = h->d;
+    rec(ev ? ev->b0 : nullptr, d.stream);
+    bh::launch_aabb(d);
+    bh::launch_keys(d, h->p.key_mode);
+    rec(ev ? ev->s0 : nullptr, d.stream);
+    bh::launch_radix_sort(d, d.n, h->p.key_mode == 1 ? 30 : 63);
+    rec(ev ? ev->s1 : nullptr, d.stream);
+    bh::launch_tie_fix(d);
+    bh::launch_permute(d);
+    bh::launch_tree_build(d, h->leaf_depth, h->p.bucket_capacity);
+    if (h->p.key_mode == 0)
+        bh::launch_tree_moments(d);
+    rec(ev ? ev->b1 : nullptr, d.stream);
+    h->have_tree = true;
+}
+
+bh::WalkParams walk_params(const bh_hip_params_t& p)
+{
+    bh::WalkParams w{};
+    w.theta2 = p.theta * p.theta;
+    w.eps2 = p.softening * p.softening;
+    w.min_d2 = p.min_distance2;
+    w.max_acc = p.max_acceleration;
+    w.criterion = p.criterion;
+    w.order = p.multipole_order;
+    w.rs = p.treepm_split_scale;
+    w.rcut2 = p.treepm_cutoff > 0.0f ? p.treepm_cutoff * p.treepm_cutoff : 0.0f;
+    return w;
+}
+
+// Queue build + walk (a force evaluation).
+void queue_force(bh_hip_handle* h)
+{
+    StepEvents* ev = next_events(h);
+    queue_build(h, ev);
+    bh::launch_walk(h->d, walk_params(h->p), h->stats);
+    rec(ev ? ev->w1 : nullptr, h->d.stream);
+    h->primed = true;
+}
+
+void begin_call(bh_hip_handle* h)
+{
+    h->events_used = 0;
+    h->integrate_ms = 0.0;
+    bh::launch_fill_u32(reinterpret_cast<uint32_t*>(h->stats), 0u, 4, h->d.stream);
+}
+
+// After a synchronised call: fold event times and counters into h->t.
+void end_call(bh_hip_handle* h)
+{
+    bh_hip_timings_t t{};
+    if (h->timing) {
+        for (size_t k = 0; k < h->events_used; ++k) {
+            const StepEvents& e = h->events[k];
+            float a = 0.0f, b = 0.0f, c = 0.0f;
+            if (hipEventElapsedTime(&a, e.b0, e.b1) == hipSuccess)
+                t.build_ms += a;
+            if (hipEventElapsedTime(&b, e.s0, e.s1) == hipSuccess)
+                t.sort_ms += b;
+            if (hipEventElapsedTime(&c, e.b1, e.w1) == hipSuccess)
+                t.walk_ms += c;
+        }
+        t.integrate_ms = h->integrate_ms;
+    }
+    unsigned long long st[2] = {0, 0};
+    if (hipMemcpy(st, h->stats, sizeof(st), hipMemcpyDeviceToHost) == hipSuccess) {
+        t.pp_interactions = st[0];
+        t.pc_interactions = st[1];
+    }
+    int info[2 * bh::kLevels + 2];
+    if (hipMemcpy(info, h->d.level_info, sizeof(info), hipMemcpyDeviceToHost) == hipSuccess) {
+        t.node_count = static_cast<uint64_t>(info[2 * bh::kLevels]);
+        t.max_level = info[2 * bh::kLevels + 1];
+    }
+    t.leaf_depth = h->leaf_depth;
+    t.walks = static_cast<int32_t>(h->events_used);
+    h->t = t;
+}
+
+bool stage_to_device(bh_hip_handle* h, const double* src3)
+{
+    return check(h,
+                 hipMemcpyAsync(h->d.stage, src3, 3 * h->d.n * sizeof(double),
+                                hipMemcpyHostToDevice, h->d.stream),
+                 "stage upload");
+}
+
+}  // namespace
+
+extern "C" {
+
+void bh_hip_default_params(bh_hip_params_t* out)
+{
+    if (!out)
+        return;
+    std::memset(out, 0, sizeof(*out));
+    out->theta = 0.5f;
+    out->softening = 0.01f;
+    out->min_softening = 1.0e-4f;   // FndConstants.hpp:22
+    out->min_distance2 = 1.0e-12f;  // FndConstants.hpp:23
+    out->min_theta = 0.05f;         // FndConstants.hpp:24
+    out->max_acceleration = 0.0f;
+    out->max_speed = 0.0f;
+    out->criterion = BH_HIP_CRITERION_COM;
+    out->leaf_capacity = 256;  // CfgConfig.hpp:63
+    out->bucket_capacity = 16;
+    out->multipole_order = 2;
+    out->key_mode = BH_HIP_KEYS_MORTON63;
+    out->integrator = BH_HIP_INTEGRATOR_LEAPFROG;
+    out->waves_per_cu = kDefaultWavesPerCu;
+}
+
+const char* bh_hip_version(void) { return "octree_hip 0.1 (gfx950)"; }
+
+bh_hip_result_t bh_hip_create(const bh_hip_params_t* params, uint64_t n, bh_hip_handle_t** out,
+                              char* err, size_t err_cap)
+{
+    if (out)
+        *out = nullptr;
+    if (!params || !out || n == 0 || n > 0x7fffffffull) {
+        set_err(err, err_cap, "bh_hip_create: invalid argument (params/out null or n out of range)");
+        return BH_HIP_INVALID_ARGUMENT;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        set_err(err, err_cap, "bh_hip_create: no HIP device available");
+        return BH_HIP_NOT_READY;
+    }
+    auto* h = new (std::nothrow) bh_hip_handle();
+    if (!h) {
+        set_err(err, err_cap, "bh_hip_create: out of host memory");
+        return BH_HIP_INTERNAL_ERROR;
+    }
+    h->p = resolve(*params);
+    int dev = 0;
+    hipDeviceProp_t prop{};
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+        prop.multiProcessorCount > 0)
+        h->cu_count = prop.multiProcessorCount;
+    h->leaf_depth = reference_leaf_depth(n, h->p.leaf_capacity);
+    bool ok = check(h, hipStreamCreateWithFlags(&h->d.stream, hipStreamNonBlocking), "stream");
+    ok = ok && alloc_all(h, n);
+    ok = ok && check(h, hipEventCreate(&h->i0), "event") && check(h, hipEventCreate(&h->i1), "event");
+    if (ok) {
+        bh::launch_init_ids(h->d);
+        ok = zero_err(h) && finish(h, "create") == BH_HIP_OK;
+    }
+    if (!ok) {
+        set_err(err, err_cap, h->last_error);
+        bh_hip_destroy(h);
+        return BH_HIP_INTERNAL_ERROR;
+    }
+    *out = h;
+    return BH_HIP_OK;
+}
+
+void bh_hip_destroy(bh_hip_handle_t* h)
+{
+    if (!h)
+        return;
+    if (h->d.stream)
+        (void)hipStreamSynchronize(h->d.stream);
+    for (void* p : h->allocations)
+        (void)hipFree(p);
+    h->allocations.clear();
+    for (auto& e : h->events) {
+        for (hipEvent_t x : {e.b0, e.s0, e.s1, e.b1, e.w1})
+            if (x)
+                (void)hipEventDestroy(x);
+    }
+    if (h->i0)
+        (void)hipEventDestroy(h->i0);
+    if (h->i1)
+        (void)hipEventDestroy(h->i1);
+    if (h->d.stream)
+        (void)hipStreamDestroy(h->d.stream);
+    delete h;
+}
+
+bh_hip_result_t bh_hip_set_params(bh_hip_handle_t* h, const bh_hip_params_t* params)
+{
+    if (!h || !params)
+        return BH_HIP_INVALID_ARGUMENT;
+    const bh_hip_params_t old = h->p;
+    h->p = resolve(*params);
+    h->p.waves_per_cu = old.waves_per_cu;  // residency is fixed at creation
+    h->leaf_depth = reference_leaf_depth(h->d.n, h->p.leaf_capacity);
+    h->primed = false;
+    h->have_tree = false;
+    return BH_HIP_OK;
+}
+
+bh_hip_result_t bh_hip_upload(bh_hip_handle_t* h, const double* pos3, const double* vel3,
+                              const float* mass)
+{
+    if (!h || !pos3 || !mass)
+        return BH_HIP_INVALID_ARGUMENT;
+    bh::Device& d = h->d;
+    // The store keeps its current order; rank maps caller index -> slot.
+    std::vector<float> m_sorted;
+    bool ok = zero_err(h) && stage_to_device(h, pos3);
+    if (ok) {
+        bh::launch_upload_scatter(d, 0);
+        ok = check(h, hipStreamSynchronize(d.stream), "upload pos");
+    }
+    if (ok) {
+        if (vel3) {
+            ok = stage_to_device(h, vel3);
+        } else {
+            ok = check(h, hipMemsetAsync(d.stage, 0, 3 * d.n * sizeof(double), d.stream),
+                       "zero velocities");
+        }
+    }
+    if (ok) {
+        bh::launch_upload_scatter(d, 1);
+        ok = check(h, hipStreamSynchronize(d.stream), "upload vel");
+    }
+    if (ok) {
+        // Masses: reorder on the host by the store's id map.
+        std::vector<uint32_t> id(d.n);
+        ok = check(h, hipMemcpy(id.data(), d.id, d.n * sizeof(uint32_t), hipMemcpyDeviceToHost),
+                   "id read");
+        if (ok) {
+            m_sorted.resize(d.n);
+            for (size_t s = 0; s < d.n; ++s)
+                m_sorted[s] = mass[id[s]];
+            ok = check(h, hipMemcpy(d.mass, m_sorted.data(), d.n * sizeof(float),
+                                    hipMemcpyHostToDevice),
+                       "mass upload");
+        }
+    }
+    h->primed = false;
+    h->have_tree = false;
+    if (!ok)
+        return BH_HIP_INTERNAL_ERROR;
+    return finish(h, "upload");
+}
+
+bh_hip_result_t bh_hip_download(bh_hip_handle_t* h, double* pos3, double* vel3)
+{
+    if (!h)
+        return BH_HIP_INVALID_ARGUMENT;
+    bh::Device& d = h->d;
+    for (int which = 0; which < 2; ++which) {
+        double* dst = which == 0 ? pos3 : vel3;
+        if (!dst)
+            continue;
+        bh::launch_download_gather(d, which);
+        if (!check(h, hipMemcpyAsync(dst, d.stage, 3 * d.n * sizeof(double),
+                                     hipMemcpyDeviceToHost, d.stream),
+                   "download") ||
+            !check(h, hipStreamSynchronize(d.stream), "download"))
+            return BH_HIP_INTERNAL_ERROR;
+    }
+    return BH_HIP_OK;
+}
+
+bh_hip_result_t bh_hip_build_tree(bh_hip_handle_t* h)
+{
+    if (!h)
+        return BH_HIP_INVALID_ARGUMENT;
+    if (!zero_err(h))
+        return BH_HIP_INTERNAL_ERROR;
+    begin_call(h);
+    queue_build(h, next_events(h));
+    const bh_hip_result_t r = finish(h, "build_tree");
+    end_call(h);
+    return r;
+}
+
+bh_hip_result_t bh_hip_compute_accelerations(bh_hip_handle_t* h, float* acc3)
+{
+    if (!h)
+        return BH_HIP_INVALID_ARGUMENT;
+    if (h->p.key_mode != 0) {
+        h->last_error = "compute_accelerations: key_mode REFERENCE30 builds topology only";
+        return BH_HIP_INVALID_ARGUMENT;
+    }
+    if (!zero_err(h))
+        return BH_HIP_INTERNAL_ERROR;
+    begin_call(h);
+    queue_force(h);
+    bh_hip_result_t r = finish(h, "compute_accelerations");
+    end_call(h);
+    if (r != BH_HIP_OK || !acc3)
+        return r;
+    bh::Device& d = h->d;
+    bh::launch_download_gather(d, 2);
+    std::vector<double> tmp(3 * d.n);
+    if (!check(h, hipMemcpyAsync(tmp.data(), d.stage, tmp.size() * sizeof(double),
+                                 hipMemcpyDeviceToHost, d.stream),
+               "acc download") ||
+        !check(h, hipStreamSynchronize(d.stream), "acc download"))
+        return BH_HIP_INTERNAL_ERROR;
+    for (size_t i = 0; i < tmp.size(); ++i)
+        acc3[i] = static_cast<float>(tmp[i]);
+    return BH_HIP_OK;
+}
+
+bh_hip_result_t bh_hip_step(bh_hip_handle_t* h, double dt, int32_t nsteps)
+{
+    if (!h || !(dt > 0.0) || nsteps < 0)
+        return BH_HIP_INVALID_ARGUMENT;
+    if (h->p.key_mode != 0) {
+        h->last_error = "step: key_mode REFERENCE30 builds topology only";
+        return BH_HIP_INVALID_ARGUMENT;
+    }
+    if (!zero_err(h))
+        return BH_HIP_INTERNAL_ERROR;
+    begin_call(h);
+    bh::Device& d = h->d;
+    const double vmax = h->p.max_speed;
+    float int_ms = 0.0f;
+    for (int k = 0; k < nsteps; ++k) {
+        if (h->p.integrator == BH_HIP_INTEGRATOR_EULER) {
+            queue_force(h);
+            rec(h->timing ? h->i0 : nullptr, d.stream);
+            bh::launch_kick_drift(d, dt, dt, vmax);
+            rec(h->timing ? h->i1 : nullptr, d.stream);
+            h->primed = false;
+        } else {
+            if (!h->primed)
+                queue_force(h);
+            rec(h->timing ? h->i0 : nullptr, d.stream);
+            bh::launch_kick_drift(d, 0.5 * dt, dt, vmax);
+            rec(h->timing ? h->i1 : nullptr, d.stream);
+            if (h->timing) {
+                // integration time of this step: the kick-drift kernel (the
+                // closing half kick is timed below in the same way)
+                if (hipEventSynchronize(h->i1) == hipSuccess &&
+                    hipEventElapsedTime(&int_ms, h->i0, h->i1) == hipSuccess)
+                    h->integrate_ms += int_ms;
+            }
+            queue_force(h);
+            rec(h->timing ? h->i0 : nullptr, d.stream);
+            bh::launch_kick(d, 0.5 * dt, vmax);
+            rec(h->timing ? h->i1 : nullptr, d.stream);
+        }
+        if (h->timing) {
+            if (hipEventSynchronize(h->i1) == hipSuccess &&
+                hipEventElapsedTime(&int_ms, h->i0, h->i1) == hipSuccess)
+                h->integrate_ms += int_ms;
+        }
+    }
+    const bh_hip_result_t r = finish(h, "step");
+    end_call(h);
+    return r;
+}
+
+bh_hip_result_t bh_hip_tree_info(bh_hip_handle_t* h, uint64_t* node_count, int32_t* leaf_depth,
+                                 int32_t* max_level)
+{
+    if (!h)
+        return BH_HIP_INVALID_ARGUMENT;
+    int info[2 * bh::kLevels + 2];
+    if (!check(h, hipStreamSynchronize(h->d.stream), "tree_info") ||
+        !check(h, hipMemcpy(info, h->d.level_info, sizeof(info), hipMemcpyDeviceToHost),
+               "tree_info"))
+        return BH_HIP_INTERNAL_ERROR;
+    if (node_count)
+        *node_count = static_cast<uint64_t>(info[2 * bh::kLevels]);
+    if (leaf_depth)
+        *leaf_depth = h->leaf_depth;
+    if (max_level)
+        *max_level = info[2 * bh::kLevels + 1];
+    return BH_HIP_OK;
+}
+
+bh_hip_result_t bh_hip_dump_tree(bh_hip_handle_t* h, uint64_t* sorted_keys, uint32_t* sorted_ids,
+                                 uint64_t* node_prefix, int32_t* node_level,
+                                 uint32_t* node_start, uint32_t* node_count,
+                                 int32_t* node_first_child, uint32_t* node_nchild,
+                                 float* node_moments, double* geom, uint64_t cap_nodes)
+{
+    if (!h)
+        return BH_HIP_INVALID_ARGUMENT;
+    if (!h->have_tree) {
+        h->last_error = "dump_tree: no tree built for the current positions";
+        return BH_HIP_NOT_READY;
+    }
+    bh::Device& d = h->d;
+    uint64_t nn = 0;
+    if (bh_hip_tree_info(h, &nn, nullptr, nullptr) != BH_HIP_OK)
+        return BH_HIP_INTERNAL_ERROR;
+    if ((node_prefix || node_level || node_start || node_count || node_first_child ||
+         node_nchild || node_moments) &&
+        cap_nodes < nn) {
+        h->last_error = "dump_tree: cap_nodes smaller than the node count";
+        return BH_HIP_INVALID_ARGUMENT;
+    }
+    auto cp = [&](void* dst, const void* src, size_t bytes, const char* what) {
+        return !dst || check(h, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), what);
+    };
+    bool ok = cp(sorted_keys, d.keys, d.n * 8, "keys") && cp(sorted_ids, d.id, d.n * 4, "ids") &&
+              cp(node_prefix, d.nprefix, nn * 8, "prefix") &&
+              cp(node_start, d.nstart, nn * 4, "start") && cp(node_count, d.ncount, nn * 4, "count") &&
+              cp(node_nchild, d.nchild, nn * 4, "nchild") &&
+              cp(node_moments, d.wnode, nn * sizeof(bh::WalkNode), "moments");
+    if (ok && node_level) {
+        std::vector<uint8_t> lv(nn);
+        ok = cp(lv.data(), d.nlevel, nn, "level");
+        for (uint64_t i = 0; ok && i < nn; ++i)
+            node_level[i] = lv[i];
+    }
+    if (ok && node_first_child) {
+        std::vector<uint32_t> f(nn), c(nn);
+        ok = cp(f.data(), d.nfirst, nn * 4, "first") && cp(c.data(), d.nchild, nn * 4, "nchild");
+        for (uint64_t i = 0; ok && i < nn; ++i)
+            node_first_child[i] = c[i] ? static_cast<int32_t>(f[i]) : -1;
+    }
+    if (ok && geom) {
+        bh::RootGeom g{};
+        ok = cp(&g, d.geom, sizeof(g), "geometry");
+        for (int a = 0; a < 3; ++a) {
+            geom[a] = g.lo[a];
+            geom[3 + a] = g.inv[a];
+            geom[6 + a] = g.lo_d[a];
+            geom[9 + a] = g.unit_d[a];
+        }
+    }
+    return ok ? BH_HIP_OK : BH_HIP_INTERNAL_ERROR;
+}
+
+bh_hip_result_t bh_hip_direct_forces_fp64(bh_hip_handle_t* h, const uint32_t* targets,
+                                          uint32_t nt, double* acc3)
+{
+    if (!h || (!targets && nt) || (!acc3 && nt))
+        return BH_HIP_INVALID_ARGUMENT;
+    if (nt == 0)
+        return BH_HIP_OK;
+    for (uint32_t i = 0; i < nt; ++i)
+        if (targets[i] >= h->d.n) {
+            h->last_error = "direct_forces_fp64: target index out of range";
+            return BH_HIP_INVALID_ARGUMENT;
+        }
+    uint32_t* dt = nullptr;
+    double* dout = nullptr;
+    bool ok = dalloc(h, &dt, nt, "targets") && dalloc(h, &dout, 3 * static_cast<size_t>(nt), "out");
+    ok = ok && check(h, hipMemcpy(dt, targets, nt * 4, hipMemcpyHostToDevice), "targets");
+    if (ok) {
+        bh::launch_direct_fp64(h->d, dt, nt, h->p.softening, h->p.min_distance2, dout);
+        ok = check(h, hipGetLastError(), "direct launch") &&
+             check(h, hipStreamSynchronize(h->d.stream), "direct") &&
+             check(h, hipMemcpy(acc3, dout, 3 * static_cast<size_t>(nt) * 8, hipMemcpyDeviceToHost),
+                   "direct out");
+    }
+    dfree(h, dt);
+    dfree(h, dout);
+    return ok ? BH_HIP_OK : BH_HIP_INTERNAL_ERROR;
+}
+
+bh_hip_result_t bh_hip_diagnostics(bh_hip_handle_t* h, uint32_t sample_limit, double* out8)
+{
+    if (!h || !out8)
+        return BH_HIP_INVALID_ARGUMENT;
+    bh::Device& d = h->d;
+    double* dres = nullptr;
+    uint32_t* dslots = nullptr;
+    double* dphi = nullptr;
+    const size_t ns = (sample_limit == 0 || d.n <= sample_limit) ? d.n : sample_limit;
+    std::vector<uint32_t> slots(ns);
+    for (size_t k = 0; k < ns; ++k)
+        slots[k] = static_cast<uint32_t>(ns == d.n ? k : (k * d.n) / ns);
+    bool ok = dalloc(h, &dres, 8, "diag out") && dalloc(h, &dslots, ns, "diag slots") &&
+              dalloc(h, &dphi, ns, "diag phi");
+    ok = ok && check(h, hipMemcpy(dslots, slots.data(), ns * 4, hipMemcpyHostToDevice), "slots");
+    std::vector<double> phi(ns);
+    double base[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (ok) {
+        bh::launch_energy(d, dres);
+        bh::launch_potential(d, dslots, static_cast<uint32_t>(ns), h->p.softening,
+                             h->p.min_distance2, dphi);
+        ok = check(h, hipGetLastError(), "diagnostics") &&
+             check(h, hipStreamSynchronize(d.stream), "diagnostics") &&
+             check(h, hipMemcpy(base, dres, sizeof(base), hipMemcpyDeviceToHost), "diag read") &&
+             check(h, hipMemcpy(phi.data(), dphi, ns * 8, hipMemcpyDeviceToHost), "phi read");
+    }
+    dfree(h, dres);
+    dfree(h, dslots);
+    dfree(h, dphi);
+    if (!ok)
+        return BH_HIP_INTERNAL_ERROR;
+    double pe = 0.0;
+    for (double v : phi)
+        pe += v;
+    pe *= 0.5 * static_cast<double>(d.n) / static_cast<double>(ns);
+    out8[0] = base[0];
+    out8[1] = pe;
+    for (int a = 0; a < 3; ++a) {
+        out8[2 + a] = base[1 + a];
+        out8[5 + a] = base[4 + a];
+    }
+    return BH_HIP_OK;
+}
+
+bh_hip_result_t bh_hip_set_timing(bh_hip_handle_t* h, int32_t enabled)
+{
+    if (!h)
+        return BH_HIP_INVALID_ARGUMENT;
+    h->timing = enabled != 0;
+    return BH_HIP_OK;
+}
+
+bh_hip_result_t bh_hip_get_timings(bh_hip_handle_t* h, bh_hip_timings_t* out)
+{
+    if (!h || !out)
+        return BH_HIP_INVALID_ARGUMENT;
+    *out = h->t;
+    return BH_HIP_OK;
+}
+
+bh_hip_result_t bh_hip_get_last_error(bh_hip_handle_t* h, char* buf, size_t cap)
+{
+    if (!h || !buf || cap == 0)
+        return BH_HIP_INVALID_ARGUMENT;
+    std::snprintf(buf, cap, "%s", h->last_error.c_str());
+    return BH_HIP_OK;
+}
+
+bh_hip_result_t bh_hip_sort_pairs(const uint64_t* keys, const uint32_t* vals, uint64_t n,
+                                  int32_t key_bits, uint64_t* keys_out, uint32_t* vals_out,
+                                  char* err, size_t err_cap)
+{
+    if ((n && (!keys || !vals || !keys_out || !vals_out)) || key_bits < 1 || key_bits > 64 ||
+        n > 0x7fffffffull) {
+        set_err(err, err_cap, "bh_hip_sort_pairs: invalid argument");
+        return BH_HIP_INVALID_ARGUMENT;
+    }
+    if (n == 0)
+        return BH_HIP_OK;
+    bh_hip_handle tmp;
+    bh::Device& d = tmp.d;
+    d.n = n;
+    bool ok = check(&tmp, hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking), "stream");
+    d.sort_hist_cap = bh::radix_hist_words(n);
+    ok = ok && dalloc(&tmp, &d.keys, n, "keys") && dalloc(&tmp, &d.keys_b, n, "keys_b") &&
+         dalloc(&tmp, &d.vals, n, "vals") && dalloc(&tmp, &d.vals_b, n, "vals_b") &&
+         dalloc(&tmp, &d.sort_hist, d.sort_hist_cap, "hist");
+    ok = ok && check(&tmp, hipMemcpy(d.keys, keys, n * 8, hipMemcpyHostToDevice), "keys in") &&
+         check(&tmp, hipMemcpy(d.vals, vals, n * 4, hipMemcpyHostToDevice), "vals in");
+    if (ok) {
+        bh::launch_radix_sort(d, n, key_bits);
+        ok = check(&tmp, hipGetLastError(), "sort") &&
+             check(&tmp, hipStreamSynchronize(d.stream), "sort") &&
+             check(&tmp, hipMemcpy(keys_out, d.keys, n * 8, hipMemcpyDeviceToHost), "keys out") &&
+             check(&tmp, hipMemcpy(vals_out, d.vals, n * 4, hipMemcpyDeviceToHost), "vals out");
+    }
+    for (void* p : tmp.allocations)
+        (void)hipFree(p);
+    tmp.allocations.clear();
+    if (d.stream)
+        (void)hipStreamDestroy(d.stream);
+    if (!ok) {
+        set_err(err, err_cap, tmp.last_error);
+        return BH_HIP_INTERNAL_ERROR;
+    }
+    return BH_HIP_OK;
+}
+
+}  // extern "C"

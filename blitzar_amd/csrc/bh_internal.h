@@ -1,0 +1,171 @@
+// Internal declarations shared by the octree_hip HIP translation units.
+// Not part of the C ABI (include/bh_hip.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace bh {
+
+constexpr int kMaxLevel = 21;          // 21 bits per axis, 63-bit keys
+constexpr int kLevels = kMaxLevel + 1;  // levels 0..21
+constexpr uint32_t kQMax = 2097151u;    // 2^21 - 1
+
+// Root geometry, written on the device once per build by geometry_kernel.
+//   FP32 part: the quantisation frame of the key kernel (AABB of the
+//   float-cast positions and its per-axis reciprocal extent).
+//   FP64 part: lattice origin and unit. A cell (level L, index i along an
+//   axis) has its centre at lattice coordinate c = (2i+1)*2^(21-L) - 1, i.e.
+//   at position lo + c * unit. Lattice coordinates are integers in
+//   [-1, 2^22 - 1), so differences of cell centres are exact in int32 and in
+//   FP32, and only the root geometry needs FP64.
+struct RootGeom {
+    float lo[3];
+    float inv[3];
+    float unit_f[3];   // FP32 copy of unit, used by the walk
+    float width_f[3];  // finest cell width = 2 * unit (FP32)
+    double lo_d[3];
+    double unit_d[3];
+};
+
+// One tree node as the walk reads it: 64 bytes, 16 words.
+//   com/mass          : centre of mass as an FP32 offset from the node's
+//                       lattice centre, total mass
+//   q                 : traceless quadrupole about the COM (xx, xy, xz, yy, yz;
+//                       zz = -xx - yy)
+//   bmax              : radius about the COM enclosing every particle
+//   size              : largest cell width at this level (opening criterion)
+//   first             : first child node (internal) or first sorted particle (leaf)
+//   cx, cy, cz        : lattice centre
+//   meta              : nchild (bits 0-3) | level (bits 4-8) | count (bits 9-31)
+struct alignas(16) WalkNode {
+    float comx, comy, comz, mass;
+    float qxx, qxy, qxz, qyy;
+    float qyz, bmax, size;
+    int first;
+    int cx, cy, cz;
+    uint32_t meta;
+};
+static_assert(sizeof(WalkNode) == 64, "WalkNode must be 64 bytes");
+
+__host__ __device__ inline uint32_t meta_nchild(uint32_t m) { return m & 15u; }
+__host__ __device__ inline uint32_t meta_level(uint32_t m) { return (m >> 4) & 31u; }
+__host__ __device__ inline uint32_t meta_count(uint32_t m) { return m >> 9; }
+__host__ __device__ inline uint32_t make_meta(uint32_t nchild, uint32_t level, uint32_t count)
+{
+    return nchild | (level << 4) | (count << 9);
+}
+constexpr uint32_t kMaxLeafCount = (1u << 23) - 1u;
+
+// Error bits raised on the device; the host reports them after a sync.
+enum DeviceError : int {
+    kErrNodeOverflow = 1,
+    kErrStackOverflow = 2,
+    kErrLeafTooLarge = 4,
+    kErrKeyMode = 8,
+};
+
+// Parameters the walk needs (by value into the kernel).
+struct WalkParams {
+    float theta2;
+    float eps2;
+    float min_d2;
+    float max_acc;      // <= 0: no clamp
+    int criterion;      // 0 = centre of mass, 1 = bounds
+    int order;          // 1 = monopole, 2 = quadrupole
+    float rs;           // TreePM split scale (0 = Newton)
+    float rcut2;        // TreePM cutoff squared (0 = none)
+};
+
+// Device buffers of one solver instance. The particle store is kept in the
+// order of the last sort (Morton order); `id` maps a store slot to the
+// caller's particle index and `rank` maps back.
+struct Device {
+    size_t n = 0;
+    size_t node_cap = 0;
+    hipStream_t stream = nullptr;
+
+    double* pos[3] = {nullptr, nullptr, nullptr};
+    double* vel[3] = {nullptr, nullptr, nullptr};
+    float* mass = nullptr;
+    uint32_t* id = nullptr;
+    double* pos_b[3] = {nullptr, nullptr, nullptr};  // permutation targets
+    double* vel_b[3] = {nullptr, nullptr, nullptr};
+    float* mass_b = nullptr;
+    uint32_t* id_b = nullptr;
+    uint32_t* rank = nullptr;  // caller index -> store slot
+    float* acc[3] = {nullptr, nullptr, nullptr};
+
+    uint64_t* keys = nullptr;
+    uint64_t* keys_b = nullptr;
+    uint32_t* vals = nullptr;
+    uint32_t* vals_b = nullptr;
+    uint32_t* sort_hist = nullptr;  // per-tile digit counts
+    size_t sort_hist_cap = 0;
+
+    float4* spart = nullptr;   // sorted particle: offset from leaf lattice centre, mass
+    uint32_t* sleaf = nullptr; // sorted particle -> leaf node
+
+    // tree (breadth-first; level L occupies [level_base[L], level_base[L]+level_count[L]))
+    uint64_t* nprefix = nullptr;
+    uint32_t* nstart = nullptr;
+    uint32_t* ncount = nullptr;
+    uint32_t* nchild = nullptr;   // child count (0 = leaf)
+    uint32_t* nfirst = nullptr;   // first child index
+    uint8_t* nlevel = nullptr;
+    WalkNode* wnode = nullptr;
+    uint32_t* scan_tmp = nullptr;  // per-level scan scratch
+    uint32_t* scan_blocks = nullptr;
+    int* level_info = nullptr;  // [0..22) base, [22..44) count, [44] total, [45] max level
+
+    RootGeom* geom = nullptr;
+    unsigned long long* aabb_bits = nullptr;  // 6 ordered-encoded doubles
+    int* err = nullptr;
+    unsigned int* work_counter = nullptr;
+    int* stack = nullptr;  // per-resident-wave traversal stacks
+    size_t stack_waves = 0;
+    size_t stack_cap = 0;
+
+    double* stage = nullptr;  // 3n doubles host<->device staging
+    double* diag = nullptr;   // reduction scratch
+    size_t diag_cap = 0;
+};
+
+// ---- launchers (defined in the kernel translation units) -------------------
+// k_keys.hip
+void launch_aabb(Device& d);
+void launch_keys(Device& d, int key_mode);
+// k_sort.hip: stable LSD radix sort of (keys, vals) pairs, 8-bit digits over
+// bits [0, key_bits). Result lands back in d.keys / d.vals.
+void launch_radix_sort(Device& d, size_t n, int key_bits);
+size_t radix_hist_words(size_t n);
+// Exclusive scan of `data` in place. The element count is *n_dev when n_dev
+// is non-null (then max_n bounds it), else n_host. The total is written to
+// total_out / total_out_i when non-null.
+void launch_scan_exclusive(uint32_t* data, size_t max_n, const int* n_dev, size_t n_host,
+                           uint32_t* block_sums, uint32_t* total_out, int* total_out_i,
+                           hipStream_t s);
+size_t scan_block_sums_words(size_t max_n);
+void launch_tie_fix(Device& d);
+void launch_permute(Device& d);
+// k_tree.hip
+void launch_tree_build(Device& d, int leaf_depth, int bucket);
+void launch_tree_moments(Device& d);
+// k_walk.hip
+// stats (optional, device): [0] particle-particle, [1] particle-cell interactions
+void launch_walk(Device& d, const WalkParams& p, unsigned long long* stats);
+// k_integrate.hip
+void launch_kick(Device& d, double dt_kick, double max_speed);
+void launch_kick_drift(Device& d, double dt_kick, double dt_drift, double max_speed);
+void launch_upload_scatter(Device& d, int which);   // 0 = pos, 1 = vel
+void launch_download_gather(Device& d, int which);  // 0 = pos, 1 = vel, 2 = acc(float->double)
+void launch_init_ids(Device& d);
+void launch_direct_fp64(Device& d, const uint32_t* targets_dev, uint32_t nt, double eps,
+                        double min_d2, double* out_dev);
+void launch_energy(Device& d, double* out_dev);
+void launch_potential(Device& d, const uint32_t* slots_dev, uint32_t ns, double eps,
+                      double min_d2, double* out_dev);
+void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+
+}  // namespace bh

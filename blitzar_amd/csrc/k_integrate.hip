@@ -1,0 +1,303 @@
+// FP64 kick/drift, host<->store reordering, FP64 direct-sum and energy
+// diagnostics for octree_hip.
+//
+// Kick/drift follows the reference KDK kernels
+// (engine/physics/core/cuda/prelude/CudParticleKernels.inl:94-220) in FP64:
+//   v += a dt/2 ; x += v dt ; <rebuild + walk> ; v += a dt/2
+// with the first force of a step reused from the end of the previous one. The
+// Euler mode is the reference CPU update
+// (engine/physics/octree/model/OctParticleSystemHost.cpp:61-69):
+//   v = clamp(v + a dt, max_speed) ; x += v dt.
+// Kick/drift traffic per particle and step: read x 24, v 24, a 12; write x 24,
+// v 24 (108 B).
+//
+// Direct sum: engine/physics/treepm/model/TpmCpu.cpp:78-116, here with FP64
+// positions (the truth the force error is measured against).
+// Energies / momenta: engine/physics/fmm/metrics/FmmMetrics.cpp:47-83.
+#include "bh_internal.h"
+
+namespace bh {
+namespace {
+
+__device__ inline void clamp3(double& x, double& y, double& z, double limit)
+{
+    const double s = sqrt(x * x + y * y + z * z);
+    if (limit > 0.0 && s > limit && s > 1e-6) {
+        const double k = limit / s;
+        x *= k;
+        y *= k;
+        z *= k;
+    }
+}
+
+__global__ __launch_bounds__(256) void kick_drift_kernel(
+    double* __restrict__ px, double* __restrict__ py, double* __restrict__ pz,
+    double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
+    const float* __restrict__ ax, const float* __restrict__ ay, const float* __restrict__ az,
+    size_t n, double dtk, double dtd, double vmax)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    double x = vx[i] + static_cast<double>(ax[i]) * dtk;
+    double y = vy[i] + static_cast<double>(ay[i]) * dtk;
+    double z = vz[i] + static_cast<double>(az[i]) * dtk;
+    clamp3(x, y, z, vmax);
+    vx[i] = x;
+    vy[i] = y;
+    vz[i] = z;
+    if (dtd != 0.0) {
+        px[i] += x * dtd;
+        py[i] += y * dtd;
+        pz[i] += z * dtd;
+    }
+}
+
+__global__ __launch_bounds__(256) void init_ids_kernel(uint32_t* __restrict__ id,
+                                                       uint32_t* __restrict__ rank, size_t n)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) {
+        id[i] = static_cast<uint32_t>(i);
+        rank[i] = static_cast<uint32_t>(i);
+    }
+}
+
+// stage holds caller-ordered xyz triples; write the store slot rank[i].
+__global__ __launch_bounds__(256) void scatter3_kernel(const double* __restrict__ stage,
+                                                       const uint32_t* __restrict__ rank,
+                                                       double* __restrict__ x,
+                                                       double* __restrict__ y,
+                                                       double* __restrict__ z, size_t n)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t s = rank[i];
+    x[s] = stage[3 * i];
+    y[s] = stage[3 * i + 1];
+    z[s] = stage[3 * i + 2];
+}
+
+__global__ __launch_bounds__(256) void gather3_kernel(double* __restrict__ stage,
+                                                      const uint32_t* __restrict__ rank,
+                                                      const double* __restrict__ x,
+                                                      const double* __restrict__ y,
+                                                      const double* __restrict__ z, size_t n)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t s = rank[i];
+    stage[3 * i] = x[s];
+    stage[3 * i + 1] = y[s];
+    stage[3 * i + 2] = z[s];
+}
+
+__global__ __launch_bounds__(256) void gather3f_kernel(double* __restrict__ stage,
+                                                       const uint32_t* __restrict__ rank,
+                                                       const float* __restrict__ x,
+                                                       const float* __restrict__ y,
+                                                       const float* __restrict__ z, size_t n)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t s = rank[i];
+    stage[3 * i] = static_cast<double>(x[s]);
+    stage[3 * i + 1] = static_cast<double>(y[s]);
+    stage[3 * i + 2] = static_cast<double>(z[s]);
+}
+
+__device__ inline double block_sum256(double v, double* red)
+{
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        red[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int k = 0; k < 4; ++k)
+        t += red[k];
+    __syncthreads();
+    return t;
+}
+
+// One 256-thread block per target: FP64 sum over every source j != target.
+__global__ __launch_bounds__(256) void direct_fp64_kernel(
+    const uint32_t* __restrict__ targets, const uint32_t* __restrict__ rank,
+    const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
+    const float* __restrict__ pm, size_t n, double eps2, double min_d2, double* __restrict__ out)
+{
+    __shared__ double red[4];
+    const uint32_t self = rank[targets[blockIdx.x]];
+    const double xi = px[self], yi = py[self], zi = pz[self];
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (size_t j = threadIdx.x; j < n; j += blockDim.x) {
+        if (j == self)
+            continue;
+        const double dx = px[j] - xi, dy = py[j] - yi, dz = pz[j] - zi;
+        const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+        if (r2 <= min_d2)
+            continue;
+        const double inv = 1.0 / sqrt(r2);
+        const double s = static_cast<double>(pm[j]) * inv * inv * inv;
+        ax += dx * s;
+        ay += dy * s;
+        az += dz * s;
+    }
+    ax = block_sum256(ax, red);
+    ay = block_sum256(ay, red);
+    az = block_sum256(az, red);
+    if (threadIdx.x == 0) {
+        out[3 * blockIdx.x] = ax;
+        out[3 * blockIdx.x + 1] = ay;
+        out[3 * blockIdx.x + 2] = az;
+    }
+}
+
+// Per-block partial sums of KE, P(3), L(3), mass.
+__global__ __launch_bounds__(256) void energy_partial_kernel(
+    const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
+    const double* __restrict__ vx, const double* __restrict__ vy, const double* __restrict__ vz,
+    const float* __restrict__ pm, size_t n, double* __restrict__ partial)
+{
+    __shared__ double red[4];
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const double m = pm[i];
+        const double x = px[i], y = py[i], z = pz[i];
+        const double u = vx[i], v = vy[i], w = vz[i];
+        acc[0] += 0.5 * m * (u * u + v * v + w * w);
+        acc[1] += m * u;
+        acc[2] += m * v;
+        acc[3] += m * w;
+        acc[4] += m * (y * w - z * v);
+        acc[5] += m * (z * u - x * w);
+        acc[6] += m * (x * v - y * u);
+        acc[7] += m;
+    }
+    for (int k = 0; k < 8; ++k) {
+        const double t = block_sum256(acc[k], red);
+        if (threadIdx.x == 0)
+            partial[8 * blockIdx.x + k] = t;
+    }
+}
+
+__global__ void energy_final_kernel(const double* __restrict__ partial, int nb,
+                                    double* __restrict__ out)
+{
+    const int k = threadIdx.x;
+    if (k >= 8)
+        return;
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b)
+        s += partial[8 * b + k];
+    out[k] = s;
+}
+
+// Softened potential of sampled targets: phi_i = -sum_{j != i} m_j / r_ij.
+__global__ __launch_bounds__(256) void potential_kernel(
+    const uint32_t* __restrict__ slots, const double* __restrict__ px,
+    const double* __restrict__ py, const double* __restrict__ pz, const float* __restrict__ pm,
+    size_t n, double eps2, double min_d2, double* __restrict__ out)
+{
+    __shared__ double red[4];
+    const uint32_t self = slots[blockIdx.x];
+    const double xi = px[self], yi = py[self], zi = pz[self];
+    double phi = 0.0;
+    for (size_t j = threadIdx.x; j < n; j += blockDim.x) {
+        if (j == self)
+            continue;
+        const double dx = px[j] - xi, dy = py[j] - yi, dz = pz[j] - zi;
+        const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+        if (r2 > min_d2)
+            phi -= static_cast<double>(pm[j]) / sqrt(r2);
+    }
+    phi = block_sum256(phi, red);
+    if (threadIdx.x == 0)
+        out[blockIdx.x] = phi * static_cast<double>(pm[self]);
+}
+
+inline unsigned blocks_for(size_t n) { return static_cast<unsigned>((n + 255) / 256); }
+
+}  // namespace
+
+void launch_kick(Device& d, double dt_kick, double max_speed)
+{
+    if (d.n == 0)
+        return;
+    kick_drift_kernel<<<blocks_for(d.n), 256, 0, d.stream>>>(
+        d.pos[0], d.pos[1], d.pos[2], d.vel[0], d.vel[1], d.vel[2], d.acc[0], d.acc[1], d.acc[2],
+        d.n, dt_kick, 0.0, max_speed);
+}
+
+void launch_kick_drift(Device& d, double dt_kick, double dt_drift, double max_speed)
+{
+    if (d.n == 0)
+        return;
+    kick_drift_kernel<<<blocks_for(d.n), 256, 0, d.stream>>>(
+        d.pos[0], d.pos[1], d.pos[2], d.vel[0], d.vel[1], d.vel[2], d.acc[0], d.acc[1], d.acc[2],
+        d.n, dt_kick, dt_drift, max_speed);
+}
+
+void launch_init_ids(Device& d)
+{
+    if (d.n == 0)
+        return;
+    init_ids_kernel<<<blocks_for(d.n), 256, 0, d.stream>>>(d.id, d.rank, d.n);
+}
+
+void launch_upload_scatter(Device& d, int which)
+{
+    if (d.n == 0)
+        return;
+    double** dst = which == 0 ? d.pos : d.vel;
+    scatter3_kernel<<<blocks_for(d.n), 256, 0, d.stream>>>(d.stage, d.rank, dst[0], dst[1],
+                                                           dst[2], d.n);
+}
+
+void launch_download_gather(Device& d, int which)
+{
+    if (d.n == 0)
+        return;
+    if (which == 2) {
+        gather3f_kernel<<<blocks_for(d.n), 256, 0, d.stream>>>(d.stage, d.rank, d.acc[0],
+                                                               d.acc[1], d.acc[2], d.n);
+        return;
+    }
+    double** src = which == 0 ? d.pos : d.vel;
+    gather3_kernel<<<blocks_for(d.n), 256, 0, d.stream>>>(d.stage, d.rank, src[0], src[1], src[2],
+                                                          d.n);
+}
+
+void launch_direct_fp64(Device& d, const uint32_t* targets_dev, uint32_t nt, double eps,
+                        double min_d2, double* out_dev)
+{
+    if (nt == 0 || d.n == 0)
+        return;
+    direct_fp64_kernel<<<nt, 256, 0, d.stream>>>(targets_dev, d.rank, d.pos[0], d.pos[1],
+                                                 d.pos[2], d.mass, d.n, eps * eps, min_d2,
+                                                 out_dev);
+}
+
+void launch_energy(Device& d, double* out_dev)
+{
+    const int nb = 256;
+    energy_partial_kernel<<<nb, 256, 0, d.stream>>>(d.pos[0], d.pos[1], d.pos[2], d.vel[0],
+                                                    d.vel[1], d.vel[2], d.mass, d.n, d.diag);
+    energy_final_kernel<<<1, 64, 0, d.stream>>>(d.diag, nb, out_dev);
+}
+
+void launch_potential(Device& d, const uint32_t* slots_dev, uint32_t ns, double eps,
+                      double min_d2, double* out_dev)
+{
+    if (ns == 0 || d.n == 0)
+        return;
+    potential_kernel<<<ns, 256, 0, d.stream>>>(slots_dev, d.pos[0], d.pos[1], d.pos[2], d.mass,
+                                               d.n, eps * eps, min_d2, out_dev);
+}
+
+}  // namespace bh

@@ -1,0 +1,421 @@
+// Stable LSD radix sort of (64-bit key, 32-bit value) pairs, device-wide
+// exclusive scan, equal-key tie ordering and the particle-store permutation.
+//
+// Contract followed: thrust::sort_by_key(keys, identity indices) at
+// engine/physics/octree/cuda/linear/OctBuild.inl:96 -- ascending keys, equal
+// keys keep ascending original particle index. The store is kept in the last
+// sorted order, so the sort's own stability orders ties by previous slot;
+// tie_fix_kernel then reorders every run of equal keys by particle id, which
+// restores the reference contract exactly.
+//
+// One pass (8-bit digit) = histogram kernel (reads 8 B/key), one scan of the
+// digit-major tile counts, and a scatter kernel that ranks keys inside a tile
+// of 2048 with wave ballots, reorders the tile in LDS and writes digit runs
+// contiguously (reads 12 B/key, writes 12 B/key).
+#include "bh_internal.h"
+
+namespace bh {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kItems = 8;
+constexpr int kTile = kBlock * kItems;  // 2048
+constexpr int kWaves = kBlock / 64;
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanChunk = kScanBlock * kScanItems;  // 2048
+
+__device__ inline unsigned long long lanemask_lt()
+{
+    const unsigned lane = threadIdx.x & 63u;
+    return lane == 0 ? 0ull : (~0ull >> (64u - lane));
+}
+
+// Inclusive scan over one wave.
+__device__ inline uint32_t wave_inclusive(uint32_t v)
+{
+    const int lane = threadIdx.x & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(v, o, 64);
+        if (lane >= o)
+            v += u;
+    }
+    return v;
+}
+
+// Exclusive scan over a 256-thread block; returns the exclusive prefix and
+// the block total in *total. `tmp` must hold kWaves words.
+__device__ inline uint32_t block_exclusive256(uint32_t v, uint32_t* tmp, uint32_t* total)
+{
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const uint32_t inc = wave_inclusive(v);
+    if (lane == 63)
+        tmp[w] = inc;
+    __syncthreads();
+    uint32_t off = 0, all = 0;
+    for (int k = 0; k < kWaves; ++k) {
+        const uint32_t s = tmp[k];
+        if (k < w)
+            off += s;
+        all += s;
+    }
+    __syncthreads();
+    *total = all;
+    return off + inc - v;
+}
+
+__global__ __launch_bounds__(kBlock) void hist_kernel(const unsigned long long* __restrict__ keys,
+                                                      size_t n, int shift,
+                                                      uint32_t* __restrict__ hist,
+                                                      uint32_t ntiles)
+{
+    __shared__ uint32_t h[256];
+    const int t = threadIdx.x;
+    h[t] = 0;
+    __syncthreads();
+    const size_t base = static_cast<size_t>(blockIdx.x) * kTile;
+    for (int k = 0; k < kItems; ++k) {
+        const size_t i = base + static_cast<size_t>(k) * kBlock + t;
+        if (i < n)
+            atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[static_cast<size_t>(t) * ntiles + blockIdx.x] = h[t];
+}
+
+__global__ __launch_bounds__(kBlock) void scatter_kernel(
+    const unsigned long long* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
+    unsigned long long* __restrict__ keys_out, uint32_t* __restrict__ vals_out, size_t n,
+    int shift, const uint32_t* __restrict__ hist, uint32_t ntiles)
+{
+    __shared__ uint32_t wcnt[kWaves][256];
+    __shared__ uint32_t tile_off[256];
+    __shared__ uint32_t gbase[256];
+    __shared__ uint32_t red[kWaves];
+    __shared__ unsigned long long sk[kTile];
+    __shared__ uint32_t sv[kTile];
+
+    const int t = threadIdx.x;
+    const int w = t >> 6;
+    const int lane = t & 63;
+    const size_t tile = blockIdx.x;
+    const size_t tbase = tile * kTile;
+    const uint32_t tile_n =
+        static_cast<uint32_t>(n - tbase < static_cast<size_t>(kTile) ? n - tbase : kTile);
+
+    for (int k = 0; k < kWaves; ++k)
+        wcnt[k][t] = 0;
+    gbase[t] = hist[static_cast<size_t>(t) * ntiles + tile];
+    __syncthreads();
+
+    unsigned long long key[kItems];
+    uint32_t val[kItems];
+    uint32_t rnk[kItems];
+    const size_t wbase = tbase + static_cast<size_t>(w) * 64 * kItems;
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        const size_t i = wbase + static_cast<size_t>(it) * 64 + lane;
+        const bool valid = i < n;
+        key[it] = valid ? keys_in[i] : 0ull;
+        val[it] = valid ? vals_in[i] : 0u;
+    }
+    const unsigned long long lt = lanemask_lt();
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        const size_t i = wbase + static_cast<size_t>(it) * 64 + lane;
+        const bool valid = i < n;
+        const uint32_t d = static_cast<uint32_t>(key[it] >> shift) & 255u;
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const unsigned long long m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t before = static_cast<uint32_t>(__popcll(peers & lt));
+        const uint32_t base = valid ? wcnt[w][d] : 0u;
+        // The lowest lane of each digit group advances that wave's counter;
+        // LDS operations of one wave complete in program order, so every
+        // lane has read the old value above.
+        if (valid && before == 0)
+            wcnt[w][d] = base + static_cast<uint32_t>(__popcll(peers));
+        rnk[it] = base + before;
+    }
+    __syncthreads();
+    // Per digit: exclusive offsets across waves, then across digits in the tile.
+    {
+        uint32_t run = 0;
+        for (int k = 0; k < kWaves; ++k) {
+            const uint32_t c = wcnt[k][t];
+            wcnt[k][t] = run;
+            run += c;
+        }
+        uint32_t total;
+        const uint32_t ex = block_exclusive256(run, red, &total);
+        tile_off[t] = ex;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        const size_t i = wbase + static_cast<size_t>(it) * 64 + lane;
+        if (i < n) {
+            const uint32_t d = static_cast<uint32_t>(key[it] >> shift) & 255u;
+            const uint32_t pos = tile_off[d] + wcnt[w][d] + rnk[it];
+            sk[pos] = key[it];
+            sv[pos] = val[it];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kItems; ++m) {
+        const uint32_t j = static_cast<uint32_t>(m) * kBlock + t;
+        if (j < tile_n) {
+            const unsigned long long k = sk[j];
+            const uint32_t d = static_cast<uint32_t>(k >> shift) & 255u;
+            const size_t out = static_cast<size_t>(gbase[d]) + (j - tile_off[d]);
+            keys_out[out] = k;
+            vals_out[out] = sv[j];
+        }
+    }
+}
+
+// ---- device-wide exclusive scan (element count from host or device) -------
+__device__ inline size_t scan_count(const int* n_dev, size_t n_host)
+{
+    return n_dev ? static_cast<size_t>(*n_dev) : n_host;
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_reduce_kernel(const uint32_t* __restrict__ in,
+                                                                 const int* n_dev, size_t n_host,
+                                                                 uint32_t* __restrict__ sums)
+{
+    __shared__ uint32_t red[kScanBlock / 64];
+    const size_t n = scan_count(n_dev, n_host);
+    const size_t base = static_cast<size_t>(blockIdx.x) * kScanChunk;
+    if (base >= n)
+        return;
+    uint32_t s = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        const size_t i = base + static_cast<size_t>(threadIdx.x) * kScanItems + k;
+        if (i < n)
+            s += in[i];
+    }
+    uint32_t total;
+    block_exclusive256(s, red, &total);
+    if (threadIdx.x == 0)
+        sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void scan_sums_kernel(uint32_t* sums, const int* n_dev,
+                                                         size_t n_host, uint32_t* total_out,
+                                                         int* total_out_i)
+{
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t carry;
+    const size_t n = scan_count(n_dev, n_host);
+    const size_t nb = (n + kScanChunk - 1) / kScanChunk;
+    const int t = threadIdx.x;
+    const int lane = t & 63, w = t >> 6;
+    if (t == 0)
+        carry = 0;
+    __syncthreads();
+    for (size_t base = 0; base < nb; base += 1024) {
+        const size_t i = base + t;
+        const uint32_t v = i < nb ? sums[i] : 0u;
+        const uint32_t inc = wave_inclusive(v);
+        if (lane == 63)
+            wsum[w] = inc;
+        __syncthreads();
+        uint32_t off = carry;
+        uint32_t all = 0;
+        for (int k = 0; k < 16; ++k) {
+            if (k < w)
+                off += wsum[k];
+            all += wsum[k];
+        }
+        if (i < nb)
+            sums[i] = off + inc - v;
+        __syncthreads();
+        if (t == 0)
+            carry += all;
+        __syncthreads();
+    }
+    if (t == 0) {
+        if (total_out)
+            *total_out = carry;
+        if (total_out_i)
+            *total_out_i = static_cast<int>(carry);
+    }
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_down_kernel(uint32_t* __restrict__ data,
+                                                               const int* n_dev, size_t n_host,
+                                                               const uint32_t* __restrict__ sums)
+{
+    __shared__ uint32_t red[kScanBlock / 64];
+    const size_t n = scan_count(n_dev, n_host);
+    const size_t base = static_cast<size_t>(blockIdx.x) * kScanChunk;
+    if (base >= n)
+        return;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        const size_t i = base + static_cast<size_t>(threadIdx.x) * kScanItems + k;
+        v[k] = i < n ? data[i] : 0u;
+        s += v[k];
+    }
+    uint32_t total;
+    uint32_t off = block_exclusive256(s, red, &total) + sums[blockIdx.x];
+    for (int k = 0; k < kScanItems; ++k) {
+        const size_t i = base + static_cast<size_t>(threadIdx.x) * kScanItems + k;
+        if (i < n)
+            data[i] = off;
+        off += v[k];
+    }
+}
+
+// Runs of equal keys: order the run by particle id (insertion sort; runs are
+// short unless particles coincide).
+__global__ __launch_bounds__(256) void tie_fix_kernel(const unsigned long long* __restrict__ keys,
+                                                      uint32_t* __restrict__ vals,
+                                                      const uint32_t* __restrict__ id, size_t n)
+{
+    const size_t s = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s + 1 >= n)
+        return;
+    const unsigned long long k = keys[s];
+    if (keys[s + 1] != k || (s > 0 && keys[s - 1] == k))
+        return;
+    size_t e = s + 2;
+    while (e < n && keys[e] == k)
+        ++e;
+    for (size_t a = s + 1; a < e; ++a) {
+        const uint32_t v = vals[a];
+        const uint32_t key_id = id[v];
+        size_t b = a;
+        while (b > s && id[vals[b - 1]] > key_id) {
+            vals[b] = vals[b - 1];
+            --b;
+        }
+        vals[b] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void permute_kernel(
+    const uint32_t* __restrict__ src, size_t n, const double* __restrict__ px,
+    const double* __restrict__ py, const double* __restrict__ pz, const double* __restrict__ vx,
+    const double* __restrict__ vy, const double* __restrict__ vz, const float* __restrict__ m,
+    const uint32_t* __restrict__ id, double* __restrict__ opx, double* __restrict__ opy,
+    double* __restrict__ opz, double* __restrict__ ovx, double* __restrict__ ovy,
+    double* __restrict__ ovz, float* __restrict__ om, uint32_t* __restrict__ oid,
+    uint32_t* __restrict__ rank)
+{
+    const size_t s = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= n)
+        return;
+    const uint32_t j = src[s];
+    opx[s] = px[j];
+    opy[s] = py[j];
+    opz[s] = pz[j];
+    ovx[s] = vx[j];
+    ovy[s] = vy[j];
+    ovz[s] = vz[j];
+    om[s] = m[j];
+    const uint32_t pid = id[j];
+    oid[s] = pid;
+    rank[pid] = static_cast<uint32_t>(s);
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, uint32_t v, size_t n)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+}
+
+}  // namespace
+
+size_t scan_block_sums_words(size_t max_n) { return (max_n + kScanChunk - 1) / kScanChunk + 1; }
+
+void launch_scan_exclusive(uint32_t* data, size_t max_n, const int* n_dev, size_t n_host,
+                           uint32_t* block_sums, uint32_t* total_out, int* total_out_i,
+                           hipStream_t s)
+{
+    const size_t cap = n_dev ? max_n : n_host;
+    const unsigned nb = static_cast<unsigned>(std::max<size_t>(1, (cap + kScanChunk - 1) / kScanChunk));
+    scan_reduce_kernel<<<nb, kScanBlock, 0, s>>>(data, n_dev, n_host, block_sums);
+    scan_sums_kernel<<<1, 1024, 0, s>>>(block_sums, n_dev, n_host, total_out, total_out_i);
+    scan_down_kernel<<<nb, kScanBlock, 0, s>>>(data, n_dev, n_host, block_sums);
+}
+
+size_t radix_hist_words(size_t n)
+{
+    const size_t tiles = std::max<size_t>(1, (n + kTile - 1) / kTile);
+    return 256 * tiles + scan_block_sums_words(256 * tiles);
+}
+
+void launch_radix_sort(Device& d, size_t n, int key_bits)
+{
+    if (n <= 1)
+        return;
+    const uint32_t ntiles = static_cast<uint32_t>((n + kTile - 1) / kTile);
+    const size_t hn = static_cast<size_t>(ntiles) * 256;
+    uint32_t* hist = d.sort_hist;
+    uint32_t* sums = d.sort_hist + hn;
+    unsigned long long* kin = reinterpret_cast<unsigned long long*>(d.keys);
+    unsigned long long* kout = reinterpret_cast<unsigned long long*>(d.keys_b);
+    uint32_t* vin = d.vals;
+    uint32_t* vout = d.vals_b;
+    const int passes = (key_bits + 7) / 8;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = 8 * p;
+        hist_kernel<<<ntiles, kBlock, 0, d.stream>>>(kin, n, shift, hist, ntiles);
+        launch_scan_exclusive(hist, hn, nullptr, hn, sums, nullptr, nullptr, d.stream);
+        scatter_kernel<<<ntiles, kBlock, 0, d.stream>>>(kin, vin, kout, vout, n, shift, hist,
+                                                        ntiles);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    if (passes & 1) {
+        // Result is in the alternate buffers: swap the roles so d.keys/d.vals hold it.
+        std::swap(d.keys, d.keys_b);
+        std::swap(d.vals, d.vals_b);
+    }
+}
+
+void launch_tie_fix(Device& d)
+{
+    if (d.n < 2)
+        return;
+    const unsigned blocks = static_cast<unsigned>((d.n + 255) / 256);
+    tie_fix_kernel<<<blocks, 256, 0, d.stream>>>(reinterpret_cast<const unsigned long long*>(d.keys),
+                                                d.vals, d.id, d.n);
+}
+
+void launch_permute(Device& d)
+{
+    if (d.n == 0)
+        return;
+    const unsigned blocks = static_cast<unsigned>((d.n + 255) / 256);
+    permute_kernel<<<blocks, 256, 0, d.stream>>>(
+        d.vals, d.n, d.pos[0], d.pos[1], d.pos[2], d.vel[0], d.vel[1], d.vel[2], d.mass, d.id,
+        d.pos_b[0], d.pos_b[1], d.pos_b[2], d.vel_b[0], d.vel_b[1], d.vel_b[2], d.mass_b, d.id_b,
+        d.rank);
+    for (int a = 0; a < 3; ++a) {
+        std::swap(d.pos[a], d.pos_b[a]);
+        std::swap(d.vel[a], d.vel_b[a]);
+    }
+    std::swap(d.mass, d.mass_b);
+    std::swap(d.id, d.id_b);
+}
+
+void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s)
+{
+    if (n == 0)
+        return;
+    fill_u32_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(p, v, n);
+}
+
+}  // namespace bh

@@ -1,0 +1,364 @@
+// Linear octree build and upward moment pass for octree_hip.
+//
+// Topology contract. The reference linear octree keeps every distinct key
+// prefix down to a fixed leaf depth D chosen from the average density
+// (engine/physics/octree/cuda/linear/OctBuild.inl:28-39, 109-199). octree_hip
+// keeps exactly those nodes (levels 0..D) and additionally splits any node
+// holding more than `bucket` particles, continuing the same prefix
+// subdivision down to level 21. So the reference tree is a truncation of this
+// tree at level D (checked bit-exactly by the tests).
+//
+// Layout: breadth-first. Level L occupies node indices
+// [base[L], base[L] + count[L]); children of one node are contiguous and in
+// octant order, so node order equals (level, prefix) order. Level counts live
+// on the device and every level kernel reads them there: the build issues no
+// host synchronisation.
+//
+// Moments (per node, FP32, relative to the node's exact lattice centre):
+// mass, centre-of-mass offset, traceless quadrupole about the COM, and bmax
+// (radius about the COM that encloses every particle of the node). Leaves
+// read their particles' FP64 positions and store FP32 offsets from the leaf's
+// FP64 lattice centre (the only FP64 arithmetic here). Parents combine
+// children with the parallel-axis theorem, using exact integer differences of
+// lattice centres.
+#include "bh_internal.h"
+
+namespace bh {
+namespace {
+
+__device__ inline uint32_t compact3(unsigned long long x)
+{
+    x &= 0x1249249249249249ull;
+    x = (x ^ (x >> 2)) & 0x10c30c30c30c30c3ull;
+    x = (x ^ (x >> 4)) & 0x100f00f00f00f00full;
+    x = (x ^ (x >> 8)) & 0x001f0000ff0000ffull;
+    x = (x ^ (x >> 16)) & 0x001f00000000ffffull;
+    x = (x ^ (x >> 32)) & 0x00000000001fffffull;
+    return static_cast<uint32_t>(x);
+}
+
+// Lattice centre of the cell with `prefix` at `level`: (2i+1)*2^(21-L) - 1.
+__device__ inline void lattice_centre(unsigned long long prefix, int level, int* c)
+{
+    const uint32_t ix = compact3(prefix >> 2), iy = compact3(prefix >> 1), iz = compact3(prefix);
+    const int sh = kMaxLevel - level;
+    c[0] = static_cast<int>(((2u * ix + 1u) << sh) - 1u);
+    c[1] = static_cast<int>(((2u * iy + 1u) << sh) - 1u);
+    c[2] = static_cast<int>(((2u * iz + 1u) << sh) - 1u);
+}
+
+__device__ inline int* level_base(int* info) { return info; }
+__device__ inline int* level_count(int* info) { return info + kLevels; }
+
+__global__ void tree_root_kernel(uint64_t* nprefix, uint8_t* nlevel, uint32_t* nstart,
+                                 uint32_t* ncount, uint32_t* nchild, uint32_t* nfirst, int* info,
+                                 uint32_t n)
+{
+    const int t = threadIdx.x;
+    if (t < kLevels) {
+        info[t] = 0;
+        info[kLevels + t] = 0;
+    }
+    if (t == 0) {
+        nprefix[0] = 0;
+        nlevel[0] = 0;
+        nstart[0] = 0;
+        ncount[0] = n;
+        nchild[0] = 0;
+        nfirst[0] = 0;
+        info[kLevels] = 1;  // count[0]
+        info[2 * kLevels] = 1;
+        info[2 * kLevels + 1] = 0;
+    }
+}
+
+// Boundaries of the 8 child octants of a node spanning sorted keys [a, b):
+// bnd[o] = first index whose level-(L+1) digit is >= o; bnd[8] = b.
+__device__ inline void child_bounds(const unsigned long long* __restrict__ keys, uint32_t a,
+                                    uint32_t b, int shift, uint32_t* bnd)
+{
+    bnd[0] = a;
+    bnd[8] = b;
+    if (b - a <= 16) {
+        uint32_t i = a;
+        for (int o = 1; o < 8; ++o) {
+            while (i < b && static_cast<int>((keys[i] >> shift) & 7u) < o)
+                ++i;
+            bnd[o] = i;
+        }
+        return;
+    }
+    uint32_t lo_prev = a;
+    for (int o = 1; o < 8; ++o) {
+        uint32_t lo = lo_prev, hi = b;
+        while (lo < hi) {
+            const uint32_t mid = lo + ((hi - lo) >> 1);
+            if (static_cast<int>((keys[mid] >> shift) & 7u) < o)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        bnd[o] = lo;
+        lo_prev = lo;
+    }
+}
+
+__global__ __launch_bounds__(256) void tree_count_kernel(
+    const unsigned long long* __restrict__ keys, const uint32_t* __restrict__ nstart,
+    const uint32_t* __restrict__ ncount, uint32_t* __restrict__ nchild,
+    uint32_t* __restrict__ nfirst, uint32_t* __restrict__ scan_tmp, int* __restrict__ info,
+    int level, int leaf_depth, uint32_t bucket)
+{
+    const int base = level_base(info)[level];
+    const int cnt = level_count(info)[level];
+    const int shift = 3 * (kMaxLevel - level - 1);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
+        const int node = base + k;
+        const uint32_t a = nstart[node], c = ncount[node];
+        const bool split = level < kMaxLevel && (level < leaf_depth || c > bucket);
+        uint32_t nc = 0;
+        if (split) {
+            uint32_t bnd[9];
+            child_bounds(keys, a, a + c, shift, bnd);
+            for (int o = 0; o < 8; ++o)
+                nc += bnd[o + 1] > bnd[o] ? 1u : 0u;
+        }
+        nchild[node] = nc;
+        nfirst[node] = a;  // leaves keep their particle start; parents are overwritten
+        scan_tmp[k] = nc;
+    }
+}
+
+// After the scan of level `level`: place level+1, clamp to capacity.
+__global__ void tree_level_fix_kernel(int* info, int level, int node_cap, int* err)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    int* base = level_base(info);
+    int* count = level_count(info);
+    const int nb = base[level] + count[level];
+    base[level + 1] = nb;
+    int c = count[level + 1];
+    if (nb + c > node_cap) {
+        // Out of node slots: level+1 is dropped and every node of `level` that
+        // wanted to split stays a leaf (see tree_emit_kernel). Forces remain
+        // correct; the host grows the capacity for the next build.
+        atomicOr(err, kErrNodeOverflow);
+        c = 0;
+        count[level + 1] = 0;
+    }
+    if (c > 0)
+        info[2 * kLevels + 1] = level + 1;  // deepest populated level
+    info[2 * kLevels] = nb + c;             // total nodes
+}
+
+__global__ __launch_bounds__(256) void tree_emit_kernel(
+    const unsigned long long* __restrict__ keys, uint64_t* __restrict__ nprefix,
+    uint8_t* __restrict__ nlevel, uint32_t* __restrict__ nstart, uint32_t* __restrict__ ncount,
+    uint32_t* __restrict__ nchild, uint32_t* __restrict__ nfirst,
+    const uint32_t* __restrict__ scan_tmp, const int* __restrict__ info, int level)
+{
+    const int base = info[level];
+    const int cnt = info[kLevels + level];
+    const int next_base = info[level + 1];
+    const bool dropped = info[kLevels + level + 1] == 0;
+    const int shift = 3 * (kMaxLevel - level - 1);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
+        const int node = base + k;
+        const uint32_t nc = nchild[node];
+        if (nc == 0)
+            continue;
+        if (dropped) {
+            nchild[node] = 0;
+            continue;
+        }
+        const uint32_t first = static_cast<uint32_t>(next_base) + scan_tmp[k];
+        nfirst[node] = first;
+        const uint32_t a = nstart[node], c = ncount[node];
+        uint32_t bnd[9];
+        child_bounds(keys, a, a + c, shift, bnd);
+        const unsigned long long p = nprefix[node];
+        uint32_t j = 0;
+        for (int o = 0; o < 8; ++o) {
+            if (bnd[o + 1] <= bnd[o])
+                continue;
+            const uint32_t child = first + j++;
+            nprefix[child] = (p << 3) | static_cast<unsigned long long>(o);
+            nlevel[child] = static_cast<uint8_t>(level + 1);
+            nstart[child] = bnd[o];
+            ncount[child] = bnd[o + 1] - bnd[o];
+            nchild[child] = 0;
+            nfirst[child] = bnd[o];
+        }
+    }
+}
+
+// Moments of one level (deepest level first). Leaves also pack their
+// particles: spart[s] = (FP32 offset from the leaf lattice centre, mass) and
+// sleaf[s] = leaf node.
+__global__ __launch_bounds__(256) void tree_moments_kernel(
+    const uint64_t* __restrict__ nprefix, const uint32_t* __restrict__ nstart,
+    const uint32_t* __restrict__ ncount, const uint32_t* __restrict__ nchild,
+    const uint32_t* __restrict__ nfirst, const int* __restrict__ info, int level,
+    const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
+    const float* __restrict__ pm, const RootGeom* __restrict__ geom, WalkNode* __restrict__ wn,
+    float4* __restrict__ spart, uint32_t* __restrict__ sleaf, int* err)
+{
+    const int base = info[level];
+    const int cnt = info[kLevels + level];
+    const float wx = geom->width_f[0], wy = geom->width_f[1], wz = geom->width_f[2];
+    const float ux = geom->unit_f[0], uy = geom->unit_f[1], uz = geom->unit_f[2];
+    const float lscale = ldexpf(1.0f, kMaxLevel - level);
+    const float size = fmaxf(fmaxf(wx, wy), wz) * lscale;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
+        const int node = base + k;
+        int c[3];
+        lattice_centre(nprefix[node], level, c);
+        const uint32_t nc = nchild[node];
+        const uint32_t first = nfirst[node];
+        float M = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+        float qxx = 0.0f, qxy = 0.0f, qxz = 0.0f, qyy = 0.0f, qyz = 0.0f, bmax = 0.0f;
+        uint32_t count = 0;
+        if (nc == 0) {
+            count = ncount[node];
+            const double ox = geom->lo_d[0] + static_cast<double>(c[0]) * geom->unit_d[0];
+            const double oy = geom->lo_d[1] + static_cast<double>(c[1]) * geom->unit_d[1];
+            const double oz = geom->lo_d[2] + static_cast<double>(c[2]) * geom->unit_d[2];
+            float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+            for (uint32_t s = first; s < first + count; ++s) {
+                const float x = static_cast<float>(px[s] - ox);
+                const float y = static_cast<float>(py[s] - oy);
+                const float z = static_cast<float>(pz[s] - oz);
+                const float m = pm[s];
+                spart[s] = make_float4(x, y, z, m);
+                sleaf[s] = static_cast<uint32_t>(node);
+                M += m;
+                sx += m * x;
+                sy += m * y;
+                sz += m * z;
+            }
+            if (M > 0.0f) {
+                cx = sx / M;
+                cy = sy / M;
+                cz = sz / M;
+            }
+            for (uint32_t s = first; s < first + count; ++s) {
+                const float4 p = spart[s];
+                const float dx = p.x - cx, dy = p.y - cy, dz = p.z - cz;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                qxx += p.w * (3.0f * dx * dx - r2);
+                qxy += p.w * (3.0f * dx * dy);
+                qxz += p.w * (3.0f * dx * dz);
+                qyy += p.w * (3.0f * dy * dy - r2);
+                qyz += p.w * (3.0f * dy * dz);
+                bmax = fmaxf(bmax, r2);
+            }
+            bmax = sqrtf(bmax);
+        } else {
+            float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+            for (uint32_t j = first; j < first + nc; ++j) {
+                const WalkNode ch = wn[j];
+                const float rx = static_cast<float>(ch.cx - c[0]) * ux + ch.comx;
+                const float ry = static_cast<float>(ch.cy - c[1]) * uy + ch.comy;
+                const float rz = static_cast<float>(ch.cz - c[2]) * uz + ch.comz;
+                M += ch.mass;
+                sx += ch.mass * rx;
+                sy += ch.mass * ry;
+                sz += ch.mass * rz;
+            }
+            if (M > 0.0f) {
+                cx = sx / M;
+                cy = sy / M;
+                cz = sz / M;
+            }
+            for (uint32_t j = first; j < first + nc; ++j) {
+                const WalkNode ch = wn[j];
+                const float dx = static_cast<float>(ch.cx - c[0]) * ux + ch.comx - cx;
+                const float dy = static_cast<float>(ch.cy - c[1]) * uy + ch.comy - cy;
+                const float dz = static_cast<float>(ch.cz - c[2]) * uz + ch.comz - cz;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                qxx += ch.qxx + ch.mass * (3.0f * dx * dx - r2);
+                qxy += ch.qxy + ch.mass * (3.0f * dx * dy);
+                qxz += ch.qxz + ch.mass * (3.0f * dx * dz);
+                qyy += ch.qyy + ch.mass * (3.0f * dy * dy - r2);
+                qyz += ch.qyz + ch.mass * (3.0f * dy * dz);
+                if (ch.mass > 0.0f)
+                    bmax = fmaxf(bmax, sqrtf(r2) + ch.bmax);
+            }
+        }
+        WalkNode w;
+        w.comx = cx;
+        w.comy = cy;
+        w.comz = cz;
+        w.mass = M;
+        w.qxx = qxx;
+        w.qxy = qxy;
+        w.qxz = qxz;
+        w.qyy = qyy;
+        w.qyz = qyz;
+        w.bmax = bmax;
+        w.size = size;
+        w.first = static_cast<int>(first);
+        w.cx = c[0];
+        w.cy = c[1];
+        w.cz = c[2];
+        if (count > kMaxLeafCount) {
+            // The walk then sees a truncated leaf: flag it; every particle was
+            // still packed above, so no access goes out of bounds.
+            atomicOr(err, kErrLeafTooLarge);
+            count = kMaxLeafCount;
+        }
+        w.meta = make_meta(nc, static_cast<uint32_t>(level), count);
+        wn[node] = w;
+    }
+}
+
+inline unsigned level_grid(size_t bound)
+{
+    const size_t b = (bound + 255) / 256;
+    return static_cast<unsigned>(std::max<size_t>(1, std::min<size_t>(b, 4096)));
+}
+
+inline size_t level_bound(int level, size_t n, size_t cap)
+{
+    size_t b = 1;
+    for (int l = 0; l < level && b < n; ++l)
+        b *= 8;
+    return std::min(std::min(b, n), cap);
+}
+
+}  // namespace
+
+void launch_tree_build(Device& d, int leaf_depth, int bucket)
+{
+    const uint32_t n = static_cast<uint32_t>(d.n);
+    tree_root_kernel<<<1, 64, 0, d.stream>>>(d.nprefix, d.nlevel, d.nstart, d.ncount, d.nchild,
+                                             d.nfirst, d.level_info, n);
+    const unsigned long long* keys = reinterpret_cast<const unsigned long long*>(d.keys);
+    const int cap = static_cast<int>(d.node_cap);
+    for (int L = 0; L < kMaxLevel; ++L) {
+        const size_t bound = level_bound(L, d.n, d.node_cap);
+        const unsigned g = level_grid(bound);
+        tree_count_kernel<<<g, 256, 0, d.stream>>>(keys, d.nstart, d.ncount, d.nchild, d.nfirst,
+                                                   d.scan_tmp, d.level_info, L, leaf_depth,
+                                                   static_cast<uint32_t>(bucket));
+        launch_scan_exclusive(d.scan_tmp, bound, d.level_info + kLevels + L, 0, d.scan_blocks,
+                              nullptr, d.level_info + kLevels + L + 1, d.stream);
+        tree_level_fix_kernel<<<1, 64, 0, d.stream>>>(d.level_info, L, cap, d.err);
+        tree_emit_kernel<<<g, 256, 0, d.stream>>>(keys, d.nprefix, d.nlevel, d.nstart, d.ncount,
+                                                  d.nchild, d.nfirst, d.scan_tmp, d.level_info,
+                                                  L);
+    }
+}
+
+void launch_tree_moments(Device& d)
+{
+    for (int L = kMaxLevel; L >= 0; --L) {
+        const size_t bound = level_bound(L, d.n, d.node_cap);
+        tree_moments_kernel<<<level_grid(bound), 256, 0, d.stream>>>(
+            d.nprefix, d.nstart, d.ncount, d.nchild, d.nfirst, d.level_info, L, d.pos[0],
+            d.pos[1], d.pos[2], d.mass, d.geom, d.wnode, d.spart, d.sleaf, d.err);
+    }
+}
+
+}  // namespace bh

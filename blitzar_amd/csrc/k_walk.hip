@@ -1,0 +1,408 @@
+// Wavefront-grouped Barnes-Hut walk for octree_hip.
+//
+// Reference semantics (per target, engine/physics/octree/traversal/
+// OctreeTraversal.cpp:129-222 and engine/physics/octree/cuda/force/
+// OctGpu.inl:100-228): skip empty nodes; a node is accepted as a single source
+// at its centre of mass when the target is not inside it and
+//   size / sqrt(|com - x|^2 + eps^2) < theta      (criterion "com"), or
+//   size / dist(x, box)               < theta      (criterion "bounds");
+// otherwise it is opened; particles of opened leaves are summed directly
+// (pair force engine/physics/octree/force/OctreeForce.cpp:23-57); the sum is
+// clamped to max_acceleration (engine/physics/octree/force/OctForces.cpp:128-133).
+//
+// MI355X mapping. One 64-lane workgroup (one wavefront) owns a group of 64
+// Morton-contiguous targets (one target per lane) and walks the tree once for
+// the whole group. A node is accepted only when the criterion holds for every
+// point of the group's bounding box (distance measured from the box, which is
+// conservative), and only when no target lies within the node's bounding
+// sphere (bmax) -- the group form of the reference's "contains self" test.
+// Accepted nodes (monopole + traceless quadrupole) and the particles of opened
+// leaves are staged in LDS lists and evaluated by all 64 lanes with broadcast
+// LDS reads. Traversal keeps a LIFO of node indices in global memory, popped
+// 64 at a time. Groups are handed out by an atomic ticket; every workgroup
+// exits when the ticket passes the group count.
+//
+// Precision: positions in the walk are FP32 offsets from the exact lattice
+// centre of the group's first leaf: float(c_node - c_ref) * unit + offset,
+// with integer lattice coordinates c, so the walk itself uses no FP64.
+//
+// Cost model for the roofline: 20 FLOP per particle-particle and 50 FLOP per
+// particle-cell (monopole + quadrupole) interaction.
+#include "bh_internal.h"
+
+namespace bh {
+namespace {
+
+constexpr int kCellCap = 128;
+constexpr int kPartCap = 256;
+
+__device__ inline float wmin(float v)
+{
+    for (int o = 32; o > 0; o >>= 1)
+        v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ inline float wmax(float v)
+{
+    for (int o = 32; o > 0; o >>= 1)
+        v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ inline uint32_t winclusive(uint32_t v)
+{
+    const int lane = threadIdx.x;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(v, o, 64);
+        if (lane >= o)
+            v += u;
+    }
+    return v;
+}
+
+// TreePM short-range weight of the Newtonian pair force:
+// w(r) = erfc(r / 2rs) + r / (rs sqrt(pi)) exp(-r^2 / 4rs^2)
+// (engine/physics/octree/force/OctreeForce.cpp:49-57).
+__device__ inline float tpm_weight(float r, float inv_rs)
+{
+    const float a = 0.5f * r * inv_rs;
+    return erfcf(a) + r * inv_rs * 0.5641895835477563f * __expf(-a * a);
+}
+
+template <int ORDER, bool TPM>
+__device__ inline void eval_cells(const float4* __restrict__ ca, const float4* __restrict__ cb,
+                                  const float2* __restrict__ cc, int ncell, float tx, float ty,
+                                  float tz, const WalkParams& P, float& ax, float& ay, float& az)
+{
+    const float inv_rs = TPM ? 1.0f / P.rs : 0.0f;
+#pragma unroll 2
+    for (int j = 0; j < ncell; ++j) {
+        const float4 a = ca[j];
+        const float dx = a.x - tx, dy = a.y - ty, dz = a.z - tz;
+        const float r2 = dx * dx + dy * dy + dz * dz + P.eps2;
+        const float ri = __frsqrt_rn(r2);
+        const float ri2 = ri * ri;
+        const float ri3 = ri * ri2;
+        float wgt = 1.0f;
+        if (TPM)
+            wgt = tpm_weight(r2 * ri, inv_rs);
+        if (ORDER >= 2) {
+            const float4 b = cb[j];
+            const float2 c = cc[j];
+            // b = (qxx, qxy, qxz, qyy), c = (qyz, qzz)
+            const float qx = b.x * dx + b.y * dy + b.z * dz;
+            const float qy = b.y * dx + b.w * dy + c.x * dz;
+            const float qz = b.z * dx + c.x * dy + c.y * dz;
+            const float dqd = dx * qx + dy * qy + dz * qz;
+            const float ri5 = ri3 * ri2;
+            const float s = (a.w * ri3 + 2.5f * dqd * ri5 * ri2) * wgt;
+            const float t = ri5 * wgt;
+            ax += s * dx - t * qx;
+            ay += s * dy - t * qy;
+            az += s * dz - t * qz;
+        } else {
+            const float s = a.w * ri3 * wgt;
+            ax += s * dx;
+            ay += s * dy;
+            az += s * dz;
+        }
+    }
+}
+
+template <bool TPM>
+__device__ inline void eval_parts(const float4* __restrict__ pl, int np, float tx, float ty,
+                                  float tz, const WalkParams& P, float& ax, float& ay, float& az)
+{
+    const float inv_rs = TPM ? 1.0f / P.rs : 0.0f;
+#pragma unroll 4
+    for (int j = 0; j < np; ++j) {
+        const float4 a = pl[j];
+        const float dx = a.x - tx, dy = a.y - ty, dz = a.z - tz;
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        const float r2 = d2 + P.eps2;
+        const float ri = __frsqrt_rn(r2);
+        bool use = r2 > P.min_d2;
+        float wgt = 1.0f;
+        if (TPM) {
+            wgt = tpm_weight(r2 * ri, inv_rs);
+            use = use && (P.rcut2 <= 0.0f || d2 <= P.rcut2);
+        }
+        const float s = use ? a.w * ri * ri * ri * wgt : 0.0f;
+        ax += s * dx;
+        ay += s * dy;
+        az += s * dz;
+    }
+}
+
+template <int ORDER, int CRIT, bool TPM>
+__global__ __launch_bounds__(64) void walk_kernel(
+    WalkParams P, const WalkNode* __restrict__ nodes, const float4* __restrict__ spart,
+    const uint32_t* __restrict__ sleaf, const RootGeom* __restrict__ geom, int n, int ngroups,
+    int node_cap, unsigned int* __restrict__ work, int* __restrict__ stack_all, int stack_cap,
+    float* __restrict__ accx, float* __restrict__ accy, float* __restrict__ accz,
+    int* __restrict__ err, unsigned long long* __restrict__ stats)
+{
+    __shared__ float4 cA[kCellCap];
+    __shared__ float4 cB[kCellCap];
+    __shared__ float2 cC[kCellCap];
+    __shared__ float4 pl[kPartCap];
+    __shared__ int lfFirst[64];
+    __shared__ uint32_t lfPre[64];
+    __shared__ float lfSx[64], lfSy[64], lfSz[64];
+    __shared__ int sGroup;
+
+    const int lane = threadIdx.x;
+    int* stk = stack_all + static_cast<size_t>(blockIdx.x) * static_cast<size_t>(stack_cap);
+    const float ux = geom->unit_f[0], uy = geom->unit_f[1], uz = geom->unit_f[2];
+    const float wfx = geom->width_f[0], wfy = geom->width_f[1], wfz = geom->width_f[2];
+    const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    unsigned long long npp = 0, npc = 0;
+
+    for (;;) {
+        if (lane == 0)
+            sGroup = static_cast<int>(atomicAdd(work, 1u));
+        __syncthreads();
+        const int g = sGroup;
+        __syncthreads();
+        if (g >= ngroups)
+            break;
+
+        const int s = g * 64 + lane;
+        const bool active = s < n;
+        const int nact = min(64, n - g * 64);
+        const int refLeaf = static_cast<int>(sleaf[g * 64]);
+        const int rcx = nodes[refLeaf].cx, rcy = nodes[refLeaf].cy, rcz = nodes[refLeaf].cz;
+
+        float tx = 0.0f, ty = 0.0f, tz = 0.0f;
+        if (active) {
+            const float4 p = spart[s];
+            const int lf = static_cast<int>(sleaf[s]);
+            tx = p.x + static_cast<float>(nodes[lf].cx - rcx) * ux;
+            ty = p.y + static_cast<float>(nodes[lf].cy - rcy) * uy;
+            tz = p.z + static_cast<float>(nodes[lf].cz - rcz) * uz;
+        }
+        const float gx0 = wmin(active ? tx : 3.0e38f), gx1 = wmax(active ? tx : -3.0e38f);
+        const float gy0 = wmin(active ? ty : 3.0e38f), gy1 = wmax(active ? ty : -3.0e38f);
+        const float gz0 = wmin(active ? tz : 3.0e38f), gz1 = wmax(active ? tz : -3.0e38f);
+        const float gcx = 0.5f * (gx0 + gx1), gcy = 0.5f * (gy0 + gy1), gcz = 0.5f * (gz0 + gz1);
+        const float ghx = 0.5f * (gx1 - gx0), ghy = 0.5f * (gy1 - gy0), ghz = 0.5f * (gz1 - gz0);
+
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+        int ncell = 0, npart = 0;
+        int sp = 1;
+        if (lane == 0)
+            stk[0] = 0;  // root
+        __syncthreads();
+
+        // Every iteration pops at least one node and a node is pushed at most
+        // once per group in a well-formed tree, so node_cap iterations bound
+        // the loop; the guard also stops a malformed tree from spinning.
+        int iters = 0;
+        while (sp > 0) {
+            if (++iters > node_cap) {
+                if (lane == 0)
+                    atomicOr(err, kErrStackOverflow);
+                break;
+            }
+            const int take = sp < 64 ? sp : 64;
+            sp -= take;
+            int node = lane < take ? stk[sp + lane] : -1;
+            if (node >= node_cap) {
+                atomicOr(err, kErrStackOverflow);
+                node = -1;
+            }
+            __syncthreads();
+
+            bool accept = false, direct = false, open = false;
+            WalkNode w = {};
+            float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+            float px = 0.0f, py = 0.0f, pz = 0.0f;
+            if (node >= 0) {
+                w = nodes[node];
+                if (w.mass > 0.0f) {
+                    sx = static_cast<float>(w.cx - rcx) * ux;
+                    sy = static_cast<float>(w.cy - rcy) * uy;
+                    sz = static_cast<float>(w.cz - rcz) * uz;
+                    px = sx + w.comx;
+                    py = sy + w.comy;
+                    pz = sz + w.comz;
+                    const float dx = fmaxf(fabsf(px - gcx) - ghx, 0.0f);
+                    const float dy = fmaxf(fabsf(py - gcy) - ghy, 0.0f);
+                    const float dz = fmaxf(fabsf(pz - gcz) - ghz, 0.0f);
+                    const float d2 = dx * dx + dy * dy + dz * dz;
+                    bool ok = d2 > w.bmax * w.bmax;
+                    if (CRIT == 1) {
+                        const int lvl = static_cast<int>(meta_level(w.meta));
+                        const float hs = ldexpf(0.5f, kMaxLevel - lvl);
+                        const float bx = fmaxf(fabsf(sx - gcx) - ghx - hs * wfx, 0.0f);
+                        const float by = fmaxf(fabsf(sy - gcy) - ghy - hs * wfy, 0.0f);
+                        const float bz = fmaxf(fabsf(sz - gcz) - ghz - hs * wfz, 0.0f);
+                        const float b2 = bx * bx + by * by + bz * bz;
+                        ok = ok && b2 > 0.0f && w.size * w.size < P.theta2 * b2;
+                    } else {
+                        ok = ok && w.size * w.size < P.theta2 * (d2 + P.eps2);
+                    }
+                    if (TPM && P.rcut2 > 0.0f) {
+                        // Prune nodes whose bounding sphere lies beyond the cutoff for the
+                        // whole group.
+                        const float lim = sqrtf(P.rcut2) + w.bmax;
+                        if (d2 > lim * lim) {
+                            ok = false;
+                            w.mass = 0.0f;
+                        }
+                    }
+                    if (w.mass > 0.0f) {
+                        const uint32_t nch = meta_nchild(w.meta);
+                        accept = ok;
+                        direct = !ok && nch == 0;
+                        open = !ok && nch != 0;
+                    }
+                }
+            }
+
+            // 1) accepted nodes -> LDS cell list
+            const unsigned long long am = __ballot(accept);
+            if (accept) {
+                const int slot = ncell + static_cast<int>(__popcll(am & lt));
+                cA[slot] = make_float4(px, py, pz, w.mass);
+                cB[slot] = make_float4(w.qxx, w.qxy, w.qxz, w.qyy);
+                cC[slot] = make_float2(w.qyz, -w.qxx - w.qyy);
+            }
+            ncell += static_cast<int>(__popcll(am));
+
+            // 2) children of opened nodes -> traversal stack
+            const uint32_t nc = open ? meta_nchild(w.meta) : 0u;
+            const uint32_t cinc = winclusive(nc);
+            const int ctot = static_cast<int>(__shfl(cinc, 63, 64));
+            if (sp + ctot > stack_cap) {
+                if (lane == 0)
+                    atomicOr(err, kErrStackOverflow);
+            } else {
+                if (open) {
+                    const int base = sp + static_cast<int>(cinc - nc);
+                    for (uint32_t j = 0; j < nc; ++j)
+                        stk[base + static_cast<int>(j)] = w.first + static_cast<int>(j);
+                }
+                sp += ctot;
+            }
+
+            // 3) opened leaves -> descriptors, then their particles -> LDS
+            const unsigned long long dm = __ballot(direct);
+            const int nd = static_cast<int>(__popcll(dm));
+            const uint32_t lc = direct ? meta_count(w.meta) : 0u;
+            const uint32_t linc = winclusive(lc);
+            const uint32_t ltot = __shfl(linc, 63, 64);
+            if (direct) {
+                const int slot = static_cast<int>(__popcll(dm & lt));
+                lfFirst[slot] = w.first;
+                lfPre[slot] = linc - lc;
+                lfSx[slot] = sx;
+                lfSy[slot] = sy;
+                lfSz[slot] = sz;
+            }
+            __syncthreads();
+
+            if (ncell > kCellCap - 64) {
+                eval_cells<ORDER, TPM>(cA, cB, cC, ncell, tx, ty, tz, P, ax, ay, az);
+                npc += static_cast<unsigned long long>(ncell);
+                ncell = 0;
+                __syncthreads();
+            }
+
+            uint32_t done = 0;
+            while (done < ltot) {
+                const uint32_t room = static_cast<uint32_t>(kPartCap - npart);
+                const uint32_t chunk = ltot - done < room ? ltot - done : room;
+                for (uint32_t f = static_cast<uint32_t>(lane); f < chunk; f += 64u) {
+                    const uint32_t gi = done + f;
+                    int lo = 0, hi = nd - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (lfPre[mid] <= gi)
+                            lo = mid;
+                        else
+                            hi = mid - 1;
+                    }
+                    const float4 q = spart[lfFirst[lo] + static_cast<int>(gi - lfPre[lo])];
+                    pl[npart + static_cast<int>(f)] =
+                        make_float4(q.x + lfSx[lo], q.y + lfSy[lo], q.z + lfSz[lo], q.w);
+                }
+                __syncthreads();
+                npart += static_cast<int>(chunk);
+                done += chunk;
+                if (npart == kPartCap) {
+                    eval_parts<TPM>(pl, npart, tx, ty, tz, P, ax, ay, az);
+                    npp += static_cast<unsigned long long>(npart);
+                    npart = 0;
+                    __syncthreads();
+                }
+            }
+        }
+        eval_cells<ORDER, TPM>(cA, cB, cC, ncell, tx, ty, tz, P, ax, ay, az);
+        eval_parts<TPM>(pl, npart, tx, ty, tz, P, ax, ay, az);
+        npc += static_cast<unsigned long long>(ncell);
+        npp += static_cast<unsigned long long>(npart);
+        // per-target interaction counts for the roofline (only active lanes are real targets)
+        npp = npp * static_cast<unsigned long long>(nact);
+        npc = npc * static_cast<unsigned long long>(nact);
+        if (stats && lane == 0) {
+            atomicAdd(&stats[0], npp);
+            atomicAdd(&stats[1], npc);
+        }
+        npp = 0;
+        npc = 0;
+        __syncthreads();
+
+        if (active) {
+            // OctHostMath.cpp:50-57
+            const float mag = sqrtf(ax * ax + ay * ay + az * az);
+            if (P.max_acc > 0.0f && mag > P.max_acc && mag > 1.0e-6f) {
+                const float k = P.max_acc / mag;
+                ax *= k;
+                ay *= k;
+                az *= k;
+            }
+            accx[s] = ax;
+            accy[s] = ay;
+            accz[s] = az;
+        }
+    }
+}
+
+template <int ORDER, int CRIT, bool TPM>
+void launch_walk_t(Device& d, const WalkParams& p, unsigned grid, unsigned long long* stats)
+{
+    const int ngroups = static_cast<int>((d.n + 63) / 64);
+    walk_kernel<ORDER, CRIT, TPM><<<grid, 64, 0, d.stream>>>(
+        p, d.wnode, d.spart, d.sleaf, d.geom, static_cast<int>(d.n), ngroups,
+        static_cast<int>(d.node_cap), d.work_counter,
+        d.stack, static_cast<int>(d.stack_cap), d.acc[0], d.acc[1], d.acc[2], d.err, stats);
+}
+
+}  // namespace
+
+void launch_walk(Device& d, const WalkParams& p, unsigned long long* stats)
+{
+    if (d.n == 0)
+        return;
+    launch_fill_u32(d.work_counter, 0u, 1, d.stream);
+    const size_t ngroups = (d.n + 63) / 64;
+    const unsigned grid = static_cast<unsigned>(std::min(ngroups, d.stack_waves));
+    const bool tpm = p.rs > 0.0f;
+    if (p.order >= 2) {
+        if (p.criterion == 1)
+            tpm ? launch_walk_t<2, 1, true>(d, p, grid, stats)
+                : launch_walk_t<2, 1, false>(d, p, grid, stats);
+        else
+            tpm ? launch_walk_t<2, 0, true>(d, p, grid, stats)
+                : launch_walk_t<2, 0, false>(d, p, grid, stats);
+    } else {
+        if (p.criterion == 1)
+            tpm ? launch_walk_t<1, 1, true>(d, p, grid, stats)
+                : launch_walk_t<1, 1, false>(d, p, grid, stats);
+        else
+            tpm ? launch_walk_t<1, 0, true>(d, p, grid, stats)
+                : launch_walk_t<1, 0, false>(d, p, grid, stats);
+    }
+}
+
+}  // namespace bh

@@ -1,0 +1,183 @@
+/*
+ * octree_hip: MI355X-native Barnes-Hut gravity solver, C ABI.
+ *
+ * This is the boundary a host (the reference's ParticleSystem with a new
+ * SolverMode::OctreeHip, or any FFI) binds. It replaces, for the octree_hip
+ * solver, the CUDA octree path that ParticleSystem::update dispatches to:
+ *   engine/physics/core/particle/PhyParticleSystem.hpp:118-123,163  (SolverMode, update)
+ *   engine/physics/core/cuda/integration/CudProfilerUpdate.inl:90-125 (dispatch)
+ *   engine/physics/octree/cuda/linear/OctBuild.inl:14-248            (buildLinearOctreeGpu)
+ *   engine/physics/octree/cuda/force/OctGpu.inl:100-276              (GPU walk)
+ *   engine/physics/core/cuda/prelude/CudParticleKernels.inl:94-220   (KDK kernels)
+ * Conventions follow the reference FFI (runtime/ffi/core/FfiApi.hpp:15-26,83-100):
+ * an opaque handle, a result enum, caller-owned error buffers, no exceptions
+ * across the boundary. Plain pointers and sizes only; all arrays are host
+ * memory owned by the caller and are copied on entry/exit. Positions and
+ * velocities are interleaved xyz triples (n x 3) in the caller's particle
+ * order; that order is preserved across calls whatever order the solver keeps
+ * internally.
+ *
+ * Threading: one handle per caller thread; handles are independent (each owns
+ * its device buffers and one HIP stream) and may coexist
+ * (engine/physics/octree/tests/reentrancy.cpp:12-35).
+ */
+#ifndef BH_HIP_H_
+#define BH_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same values as blitzar_core_result_t (runtime/ffi/core/FfiApi.hpp:15-21). */
+typedef enum bh_hip_result {
+    BH_HIP_OK = 0,
+    BH_HIP_INVALID_ARGUMENT = 1,
+    BH_HIP_NOT_READY = 2,
+    BH_HIP_TIMEOUT = 3,
+    BH_HIP_INTERNAL_ERROR = 4
+} bh_hip_result_t;
+
+enum { BH_HIP_INTEGRATOR_LEAPFROG = 0, BH_HIP_INTEGRATOR_EULER = 1 };
+enum { BH_HIP_CRITERION_COM = 0, BH_HIP_CRITERION_BOUNDS = 1 };
+enum { BH_HIP_KEYS_MORTON63 = 0, BH_HIP_KEYS_REFERENCE30 = 1 };
+
+/*
+ * Solver configuration: the setters ParticleSystem receives from
+ * configureSystem (engine/batch/runner/BatRunner.cpp:48-97). theta and
+ * softening are resolved like resolveForceLawPolicy
+ * (engine/physics/core/force/PhyForceLawPolicy.cpp:22-32):
+ *   softening = max(softening, max(0, min_softening))
+ *   theta     = clamp(theta, clamp(min_theta, 0.05, 4), 4)
+ */
+typedef struct bh_hip_params {
+    float theta;             /* octree(theta) */
+    float softening;         /* octree(softening): Plummer epsilon */
+    float min_softening;     /* physics(min_softening) */
+    float min_distance2;     /* physics(min_distance2): pairs closer are skipped */
+    float min_theta;         /* physics(min_theta) */
+    float max_acceleration;  /* physics(max_acceleration); <= 0 disables the clamp */
+    float max_speed;         /* sph(max_speed) velocity clamp; <= 0 disables */
+    int32_t criterion;       /* BH_HIP_CRITERION_* (octree(criterion)) */
+    int32_t leaf_capacity;   /* octree(leaf_capacity): reference leaf depth rule */
+    int32_t bucket_capacity; /* max particles per leaf after refinement (>= 1) */
+    int32_t multipole_order; /* 1 = monopole, 2 = monopole + quadrupole */
+    int32_t key_mode;        /* BH_HIP_KEYS_*; REFERENCE30 builds topology only */
+    int32_t integrator;      /* BH_HIP_INTEGRATOR_* */
+    float treepm_split_scale;/* r_s of the erfc short-range split; 0 = Newton */
+    float treepm_cutoff;     /* short-range cutoff radius; 0 = none */
+    int32_t waves_per_cu;    /* walk residency per CU (0 = default) */
+    int32_t reserved[7];
+} bh_hip_params_t;
+
+typedef struct bh_hip_handle bh_hip_handle_t;
+
+/* Per-phase device time summed over the force evaluations of the last call
+ * (milliseconds, HIP events on the handle's stream; enable with
+ * bh_hip_set_timing) and walk interaction counts. */
+typedef struct bh_hip_timings {
+    double build_ms;      /* AABB + keys + sort + permute + tree + moments */
+    double sort_ms;       /* radix sort alone */
+    double walk_ms;       /* tree walk */
+    double integrate_ms;  /* kick/drift kernels */
+    uint64_t pp_interactions; /* particle-particle interactions (sum over timed walks) */
+    uint64_t pc_interactions; /* particle-cell interactions (sum over timed walks) */
+    uint64_t node_count;      /* nodes of the last tree */
+    int32_t leaf_depth;       /* reference leaf depth of the last tree */
+    int32_t max_level;        /* deepest populated level of the last tree */
+    int32_t walks;            /* force evaluations the sums above cover */
+    int32_t reserved;
+} bh_hip_timings_t;
+
+/* Fill `out` with the defaults (reference config defaults, CfgConfig.hpp). */
+void bh_hip_default_params(bh_hip_params_t* out);
+
+/* Create a solver for n particles on the current HIP device. On failure
+ * returns an error, writes a message into err (when non-NULL) and sets *out
+ * to NULL. */
+bh_hip_result_t bh_hip_create(const bh_hip_params_t* params, uint64_t n, bh_hip_handle_t** out,
+                              char* err, size_t err_cap);
+void bh_hip_destroy(bh_hip_handle_t* h);
+
+/* Replace the configuration (same resolution rules as create). */
+bh_hip_result_t bh_hip_set_params(bh_hip_handle_t* h, const bh_hip_params_t* params);
+
+/* Copy state in (caller order). vel3 may be NULL (zero velocities). */
+bh_hip_result_t bh_hip_upload(bh_hip_handle_t* h, const double* pos3, const double* vel3,
+                              const float* mass);
+/* Copy state out (caller order). Either pointer may be NULL. */
+bh_hip_result_t bh_hip_download(bh_hip_handle_t* h, double* pos3, double* vel3);
+
+/* Build the tree for the current positions and compute accelerations
+ * (the force pass of ParticleSystem::update). acc3 (caller order, n x 3) may
+ * be NULL to keep the result on the device only. */
+bh_hip_result_t bh_hip_compute_accelerations(bh_hip_handle_t* h, float* acc3);
+
+/* Advance nsteps steps of size dt with the configured integrator
+ * (leapfrog: kick-drift-kick reusing the previous force; euler: the
+ * reference CPU update v = clamp(v + a dt), x += v dt). */
+bh_hip_result_t bh_hip_step(bh_hip_handle_t* h, double dt, int32_t nsteps);
+
+/* Build the tree only (keys, sort, topology, moments) for the current
+ * positions; used for topology checks. */
+bh_hip_result_t bh_hip_build_tree(bh_hip_handle_t* h);
+
+/* Number of nodes of the last built tree, reference leaf depth and the
+ * deepest level; any pointer may be NULL. */
+bh_hip_result_t bh_hip_tree_info(bh_hip_handle_t* h, uint64_t* node_count, int32_t* leaf_depth,
+                                 int32_t* max_level);
+
+/* Dump the last built tree. Arrays sized n (particles) or cap_nodes (nodes);
+ * any pointer may be NULL.
+ *   sorted_keys/sorted_ids : keys in sorted order and the caller index at each
+ *                            sorted position
+ *   node_prefix/level/start/count/first_child/nchild : breadth-first nodes
+ *                            (first_child = -1 for leaves)
+ *   node_moments           : 16 floats per node: com offset xyz, mass,
+ *                            qxx qxy qxz qyy qyz, bmax, size, (first),
+ *                            lattice centre xyz, (meta)  (raw 64-byte records)
+ *   geom                   : 12 doubles: FP32 frame lo[3], inv[3]; lattice
+ *                            origin[3], unit[3] */
+bh_hip_result_t bh_hip_dump_tree(bh_hip_handle_t* h, uint64_t* sorted_keys, uint32_t* sorted_ids,
+                                 uint64_t* node_prefix, int32_t* node_level,
+                                 uint32_t* node_start, uint32_t* node_count,
+                                 int32_t* node_first_child, uint32_t* node_nchild,
+                                 float* node_moments, double* geom, uint64_t cap_nodes);
+
+/* FP64 direct sum on sampled targets (caller indices), against all n
+ * sources, with the configured softening and min_distance2
+ * (engine/physics/treepm/model/TpmCpu.cpp:78-116). acc3 is nt x 3. */
+bh_hip_result_t bh_hip_direct_forces_fp64(bh_hip_handle_t* h, const uint32_t* targets,
+                                          uint32_t nt, double* acc3);
+
+/* FP64 diagnostics (engine/physics/fmm/metrics/FmmMetrics.cpp:47-83):
+ * out[0] kinetic energy, out[1] potential energy (exact when n <= sample_limit,
+ * otherwise estimated from sample_limit evenly spaced particles), out[2..4]
+ * linear momentum, out[5..7] angular momentum. */
+bh_hip_result_t bh_hip_diagnostics(bh_hip_handle_t* h, uint32_t sample_limit, double* out8);
+
+/* Timings: enable (1) or disable (0) per-phase event timing; read the last
+ * values. */
+bh_hip_result_t bh_hip_set_timing(bh_hip_handle_t* h, int32_t enabled);
+bh_hip_result_t bh_hip_get_timings(bh_hip_handle_t* h, bh_hip_timings_t* out);
+
+/* Last error message of the handle (FfiApi.hpp:83-100 style). */
+bh_hip_result_t bh_hip_get_last_error(bh_hip_handle_t* h, char* buf, size_t cap);
+
+/* Standalone stable radix sort of (key, value) pairs on the device, the
+ * building block behind thrust::sort_by_key at OctBuild.inl:96. Sorts by the
+ * low key_bits bits (1..64); host arrays in and out. */
+bh_hip_result_t bh_hip_sort_pairs(const uint64_t* keys, const uint32_t* vals, uint64_t n,
+                                  int32_t key_bits, uint64_t* keys_out, uint32_t* vals_out,
+                                  char* err, size_t err_cap);
+
+/* Library version string. */
+const char* bh_hip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BH_HIP_H_ */

@@ -1,0 +1,80 @@
+"""The C-ABI library loads and exports every function include/*.h declares.
+Runs without a GPU: no compute call is made except the host-only IC
+generators and the clean failure of bh_hip_create when no device exists."""
+import ctypes as C
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+
+def declared_functions():
+    names = set()
+    for path in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(path).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(bh_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_headers_declare_entry_points():
+    names = declared_functions()
+    assert "bh_hip_create" in names and "bh_hip_step" in names and "bh_ic_plummer" in names
+
+
+def test_library_exports_every_declared_symbol():
+    import blitzar_amd
+    lib = blitzar_amd.lib()
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(declared_functions()) <= set(blitzar_amd.EXPORTS)
+
+
+def test_version_and_defaults():
+    import blitzar_amd
+    assert b"octree_hip" in blitzar_amd.lib().bh_hip_version()
+    p = blitzar_amd.default_params()
+    assert p.theta == pytest.approx(0.5) and p.leaf_capacity == 256 and p.multipole_order == 2
+
+
+def test_create_rejects_bad_arguments_without_device():
+    import blitzar_amd
+    from blitzar_amd import _abi
+    h = C.c_void_p()
+    err = C.create_string_buffer(256)
+    p = blitzar_amd.default_params()
+    rc = blitzar_amd.lib().bh_hip_create(C.byref(p), 0, C.byref(h), err, 256)
+    assert rc == _abi.INVALID_ARGUMENT and not h.value
+    assert b"invalid" in err.value
+
+
+def test_initial_conditions_deterministic_and_centred():
+    import blitzar_amd as B
+    p1, v1, m1 = B.plummer(20000, 42)
+    p2, v2, m2 = B.plummer(20000, 42)
+    assert np.array_equal(p1, p2) and np.array_equal(v1, v2)
+    assert abs(m1.sum() - 1.0) < 1e-6
+    assert np.abs((m1[:, None] * p1).sum(0)).max() < 1e-12
+    # Plummer virial ratio 2K/|W| ~ 1 (W = -3 pi / 32 for G = M = a = 1)
+    ke = 0.5 * (m1 * (v1 ** 2).sum(1)).sum()
+    assert ke == pytest.approx(3 * np.pi / 64, rel=0.03)
+    # half-mass radius of the Plummer sphere: a / sqrt(2^(2/3) - 1)
+    r = np.linalg.norm(p1, axis=1)
+    assert np.median(r) == pytest.approx(1 / np.sqrt(2 ** (2 / 3) - 1), rel=0.03)
+
+
+def test_hernquist_profile():
+    import blitzar_amd as B
+    p, v, m = B.hernquist(20000, 11)
+    r = np.linalg.norm(p, axis=1)
+    # half-mass radius (1 + sqrt 2) a for the untruncated profile (r_max = 200 a
+    # holds 99 % of the mass)
+    assert np.median(r) == pytest.approx(1 + np.sqrt(2), rel=0.05)
+    ke = 0.5 * (m * (v ** 2).sum(1)).sum()
+    # K = |W| / 2 = 1 / 12 (G = M = a = 1)
+    assert ke == pytest.approx(1 / 12, rel=0.06)

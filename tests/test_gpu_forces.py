@@ -1,0 +1,191 @@
+"""GPU parity of the walk (forces), the FP64 direct sum, kick/drift and the
+FP64 diagnostics, through the C ABI.
+
+Tolerances (FP32 walk):
+  * near-exact opening (theta = 0.05): median relative error vs FP64 direct
+    sum <= 1e-5 (FP32 rounding only)
+  * theta = 0.5 quadrupole: median <= 1e-3 (the metric's bar), P99 <= 1e-2
+  * theta = 0.5 vs the octree_cpu restatement at the same theta: median
+    relative difference <= 1e-3 (octree_cpu is monopole-only, so the
+    difference is dominated by its own error)
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, read_xyz
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-30)
+
+
+def test_two_body_closed_form():
+    import blitzar_amd as B
+    say("case two_body")
+    pos, mass = read_xyz(os.path.join(GOLDEN, "two_body_rest.xyz"))
+    with B.OctreeHip(2, softening=0.01) as s:
+        s.upload(pos, None, mass.astype(np.float32))
+        a = s.compute_accelerations()
+    d = 2.0
+    expect = d / (d * d + 1e-4) ** 1.5
+    assert a[0, 0] == pytest.approx(expect, rel=2e-6)
+    assert a[1, 0] == pytest.approx(-expect, rel=2e-6)
+    assert np.all(a[:, 1:] == 0.0)
+
+
+def test_single_particle_has_no_force():
+    import blitzar_amd as B
+    with B.OctreeHip(1) as s:
+        s.upload(np.array([[0.3, -0.2, 5.0]]), None, np.array([1.0], np.float32))
+        a = s.compute_accelerations()
+    assert np.all(a == 0.0)
+
+
+def test_direct_fp64_matches_oracle(oracle):
+    import blitzar_amd as B
+    n = 20000
+    say(f"case direct n={n}")
+    pos, vel, m = B.plummer(n, 3)
+    t = np.arange(0, n, 97, dtype=np.uint32)
+    with B.OctreeHip(n, softening=0.01) as s:
+        s.upload(pos, vel, m)
+        a = s.direct_forces_fp64(t)
+    ref = oracle.direct_fp64(pos, m.astype(np.float64), t.astype(np.int32), eps=0.01,
+                             min_d2=float(np.float32(1e-12)))
+    assert rel(a, ref).max() < 1e-12
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_small_theta_is_direct_sum(oracle, order):
+    import blitzar_amd as B
+    n = 8000
+    say(f"case theta0.05 order={order}")
+    pos, vel, m = B.plummer(n, 9)
+    with B.OctreeHip(n, theta=0.05, softening=0.01, multipole_order=order,
+                     leaf_capacity=16, bucket_capacity=16) as s:
+        s.upload(pos, vel, m)
+        a = s.compute_accelerations()
+    t = np.arange(0, n, 7)
+    ref = oracle.direct_fp64(pos, m.astype(np.float64), t, eps=0.01)
+    r = rel(a[t], ref)
+    assert np.median(r) < 1e-5 and r.max() < 1e-3
+
+
+@pytest.mark.parametrize("criterion", ["com", "bounds"])
+@pytest.mark.parametrize("n", [5000, 60000])
+def test_theta_half_accuracy(oracle, criterion, n):
+    import blitzar_amd as B
+    say(f"case theta0.5 n={n} criterion={criterion}")
+    pos, vel, m = B.plummer(n, 7)
+    with B.OctreeHip(n, theta=0.5, softening=0.005, criterion=criterion) as s:
+        s.upload(pos, vel, m)
+        a = s.compute_accelerations()
+        tm = s.timings()
+    rng = np.random.default_rng(1)
+    t = rng.choice(n, 1024, replace=False)
+    ref = oracle.direct_fp64(pos, m.astype(np.float64), t, eps=0.005)
+    r = rel(a[t], ref)
+    say(f"  median={np.median(r):.3e} p99={np.percentile(r, 99):.3e} "
+        f"pp={tm['pp_interactions']} pc={tm['pc_interactions']}")
+    assert np.median(r) < 1e-3
+    assert np.percentile(r, 99) < 1e-2
+    # quadrupole beats the monopole-only octree_cpu at the same theta
+    x = oracle.hot(pos, m)
+    acpu = oracle.octree_forces_subset(x, t, oracle.make_params(theta=0.5, softening=0.005,
+                                                                criterion=1 if criterion == "bounds" else 0))
+    rc = rel(acpu, ref)
+    assert np.median(r) <= np.median(rc)
+    assert np.median(rel(a[t], acpu)) < max(1e-3, 1.5 * np.median(rc))
+
+
+def test_clamp_and_min_distance(oracle):
+    import blitzar_amd as B
+    pos = np.array([[0.0, 0, 0], [1e-3, 0, 0], [5.0, 0, 0]])
+    m = np.ones(3, np.float32)
+    with B.OctreeHip(3, softening=1e-4, max_acceleration=64.0) as s:
+        s.upload(pos, None, m)
+        a = s.compute_accelerations()
+    assert np.linalg.norm(a[0]) == pytest.approx(64.0, rel=1e-5)
+    # coincident pair with zero softening: skipped by min_distance2, no NaN
+    pos = np.array([[0.0, 0, 0], [0.0, 0, 0], [1.0, 0, 0]])
+    with B.OctreeHip(3, softening=0.0, min_softening=0.0, min_distance2=1e-12) as s:
+        s.upload(pos, None, m)
+        a = s.compute_accelerations()
+    assert np.all(np.isfinite(a))
+    assert a[0, 0] == pytest.approx(1.0, rel=1e-5)
+
+
+def test_kdk_energy_and_reversibility():
+    import blitzar_amd as B
+    n = 4096
+    say(f"case kdk n={n}")
+    pos, vel, m = B.plummer(n, 42)
+    with B.OctreeHip(n, theta=0.4, softening=0.01, integrator="leapfrog") as s:
+        s.upload(pos, vel, m)
+        e0 = s.diagnostics()
+        s.step(1e-3, 50)
+        e1 = s.diagnostics()
+        p1, v1 = s.download()
+        # time reversal: flip velocities, step back
+        s.upload(p1, -v1, m)
+        s.step(1e-3, 50)
+        p2, v2 = s.download()
+    E0 = e0["ke"] + e0["pe"]
+    E1 = e1["ke"] + e1["pe"]
+    assert abs(E1 - E0) / abs(E0) < 1e-4
+    assert np.abs(p2 - pos).max() < 1e-6
+    assert np.abs(e1["p"]).max() < 1e-6
+
+
+def test_euler_matches_reference_update():
+    import blitzar_amd as B
+    pos = np.array([[-1.0, 0, 0], [1.0, 0, 0]])
+    m = np.ones(2, np.float32)
+    dt = 0.01
+    with B.OctreeHip(2, softening=0.01, integrator="euler") as s:
+        s.upload(pos, None, m)
+        a = s.compute_accelerations()
+        assert s.update(dt)
+        p, v = s.download()
+    assert v[0, 0] == pytest.approx(a[0, 0] * dt, rel=1e-6)
+    assert p[0, 0] == pytest.approx(-1.0 + a[0, 0] * dt * dt, rel=1e-6)
+
+
+def test_diagnostics_match_oracle(oracle):
+    import blitzar_amd as B
+    n = 3000
+    pos, vel, m = B.plummer(n, 2)
+    with B.OctreeHip(n, softening=0.01) as s:
+        s.upload(pos, vel, m)
+        d = s.diagnostics()
+        ds = s.diagnostics(sample_limit=500)
+    ref = oracle.energy(pos, vel, m.astype(np.float64), 0.01, float(np.float32(1e-12)))
+    assert d["ke"] == pytest.approx(ref["ke"], rel=1e-12)
+    assert d["pe"] == pytest.approx(ref["pe"], rel=1e-10)
+    assert np.allclose(d["p"], ref["p"], atol=1e-12)
+    assert np.allclose(d["l"], ref["l"], atol=1e-12)
+    assert ds["pe"] == pytest.approx(ref["pe"], rel=0.1)
+
+
+def test_handles_coexist():
+    import blitzar_amd as B
+    a_pos, a_vel, a_m = B.plummer(3000, 1)
+    b_pos, b_vel, b_m = B.plummer(5000, 2)
+    with B.OctreeHip(3000) as s1, B.OctreeHip(5000) as s2:
+        s1.upload(a_pos, a_vel, a_m)
+        s2.upload(b_pos, b_vel, b_m)
+        x1 = s1.compute_accelerations()
+        x2 = s2.compute_accelerations()
+        y1 = s1.compute_accelerations()
+    assert np.array_equal(x1, y1)
+    assert x2.shape == (5000, 3)

@@ -1,0 +1,163 @@
+"""GPU parity of keys, sort order and tree topology (bit-exact) and of the
+node moments (FP32 vs the oracle's FP64), through the C ABI."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def _solver(n, **kw):
+    import blitzar_amd as B
+    return B.OctreeHip(n, **kw)
+
+
+def _clustered(n, seed):
+    rng = np.random.default_rng(seed)
+    pos = rng.standard_normal((n, 3)) ** 3 * np.array([1.0, 0.7, 1.3])
+    m = (0.5 + rng.random(n)).astype(np.float32) / n
+    return pos, m
+
+
+@pytest.mark.parametrize("n,key_mode", [(1, 0), (2, 0), (63, 0), (64, 0), (65, 0), (3000, 0),
+                                        (50000, 0), (3000, 1), (50000, 1)])
+def test_keys_and_sorted_order_bit_exact(oracle, n, key_mode):
+    say(f"case keys n={n} key_mode={key_mode}")
+    pos, m = _clustered(n, 100 + n)
+    with _solver(n, key_mode=key_mode, leaf_capacity=16, bucket_capacity=8) as s:
+        s.upload(pos, None, m)
+        s.build_tree()
+        t = s.dump_tree()
+    k = oracle.keys(pos, key_mode)
+    sk, order = oracle.stable_sort(k)
+    assert np.array_equal(t["sorted_keys"], sk)
+    assert np.array_equal(t["sorted_ids"], order)
+
+
+def test_sorted_order_ties_by_particle_index(oracle):
+    # Coincident particles tie on the key; after a step the store is in Morton
+    # order, and the tie order must still follow the caller index.
+    n = 4000
+    say(f"case ties n={n}")
+    rng = np.random.default_rng(7)
+    base = rng.standard_normal((n // 4, 3))
+    pos = np.repeat(base, 4, axis=0)[rng.permutation(n)]
+    m = np.full(n, 1.0 / n, np.float32)
+    with _solver(n, leaf_capacity=16, bucket_capacity=8) as s:
+        s.upload(pos, None, m)
+        s.build_tree()
+        s.build_tree()  # second build starts from the permuted store
+        t = s.dump_tree()
+    sk, order = oracle.stable_sort(oracle.keys(pos, 0))
+    assert np.array_equal(t["sorted_keys"], sk)
+    assert np.array_equal(t["sorted_ids"], order)
+
+
+@pytest.mark.parametrize("n,leaf_cap,bucket", [(1, 256, 16), (5, 256, 16), (3000, 16, 8),
+                                               (40000, 256, 16), (40000, 16, 1), (40000, 64, 64)])
+def test_topology_bit_exact(oracle, n, leaf_cap, bucket):
+    say(f"case topology n={n} leaf_cap={leaf_cap} bucket={bucket}")
+    pos, m = _clustered(n, 200 + n)
+    with _solver(n, leaf_capacity=leaf_cap, bucket_capacity=bucket) as s:
+        s.upload(pos, None, m)
+        s.build_tree()
+        t = s.dump_tree()
+    sk, _ = oracle.stable_sort(oracle.keys(pos, 0))
+    d = oracle.leaf_depth(n, leaf_cap)
+    assert t["leaf_depth"] == d
+    ref = oracle.hip_tree(sk, d, bucket)
+    assert len(t["level"]) == len(ref["level"])
+    for key in ("level", "prefix", "start", "count", "first_child", "nchild"):
+        assert np.array_equal(t[key].astype(np.int64), ref[key].astype(np.int64)), key
+
+
+@pytest.mark.parametrize("key_mode", [0, 1])
+def test_reference_tree_is_truncation(oracle, key_mode):
+    n = 30000
+    say(f"case truncation key_mode={key_mode}")
+    pos, m = _clustered(n, 31)
+    with _solver(n, key_mode=key_mode, leaf_capacity=16, bucket_capacity=8) as s:
+        s.upload(pos, None, m)
+        s.build_tree()
+        t = s.dump_tree()
+    sk, order = oracle.stable_sort(oracle.keys(pos, key_mode))
+    d = oracle.leaf_depth(n, 16)
+    ref = oracle.reference_tree(sk, order, pos, m, d)
+    top = int(ref["level"].min())
+    rset = {(int(a), int(b), int(c), int(e)) for a, b, c, e in
+            zip(ref["level"], ref["prefix"], ref["start"], ref["count"])}
+    hset = {(int(a), int(b), int(c), int(e)) for a, b, c, e in
+            zip(t["level"], t["prefix"], t["start"], t["count"]) if top <= a <= d}
+    assert rset == hset
+    # FP64 subtree masses equal the reference node masses
+    lvl = t["level"]
+    sel = (lvl >= top) & (lvl <= d)
+    mine = {(int(a), int(b)): float(mm) for a, b, mm in
+            zip(lvl[sel], t["prefix"][sel], t["moments"][sel, 3])}
+    for a, b, mm in zip(ref["level"], ref["prefix"], ref["msum"]):
+        if key_mode == 0:
+            assert mine[(int(a), int(b))] == pytest.approx(mm, rel=2e-5)
+
+
+def test_moments_match_fp64(oracle):
+    n = 20000
+    say(f"case moments n={n}")
+    import blitzar_amd as B
+    pos, vel, m = B.plummer(n, 5)
+    with _solver(n, leaf_capacity=16, bucket_capacity=16) as s:
+        s.upload(pos, vel, m)
+        s.build_tree()
+        t = s.dump_tree()
+    g = t["geom"]
+    lo, unit = g[6:9], g[9:12]
+    mo = t["moments"]
+    lat = t["lattice"].astype(np.float64)
+    ids = t["sorted_ids"]
+    rng = np.random.default_rng(0)
+    nodes = rng.choice(len(t["level"]), 300, replace=False)
+    nodes = np.unique(np.concatenate([nodes, [0]]))
+    for i in nodes:
+        members = ids[t["start"][i]:t["start"][i] + t["count"][i]]
+        ref = oracle.moments(pos[members], m[members].astype(np.float64))
+        com = lo + lat[i] * unit + mo[i, 0:3]
+        assert mo[i, 3] == pytest.approx(ref[0], rel=1e-5)
+        size = mo[i, 10]
+        assert np.abs(com - ref[1:4]).max() <= 1e-5 * size + 1e-12
+        q = np.array([mo[i, 4], mo[i, 5], mo[i, 6], mo[i, 7], mo[i, 8], -mo[i, 4] - mo[i, 7]])
+        qref = ref[[4, 5, 6, 7, 8, 9]]
+        scale = ref[0] * size * size
+        assert np.abs(q - qref).max() <= 2e-4 * scale + 1e-12
+        # bmax encloses every particle
+        r = np.linalg.norm(pos[members] - ref[1:4], axis=1).max()
+        assert mo[i, 9] >= r * (1 - 1e-5) - 1e-6 * size
+
+
+def test_sort_pairs_edge_cases():
+    import blitzar_amd as B
+    rng = np.random.default_rng(11)
+    cases = [
+        ("single", np.array([5], np.uint64)),
+        ("pair", np.array([9, 3], np.uint64)),
+        ("tile-1", rng.integers(0, 2 ** 63, 2047, dtype=np.uint64)),
+        ("tile", rng.integers(0, 2 ** 63, 2048, dtype=np.uint64)),
+        ("tile+1", rng.integers(0, 2 ** 63, 2049, dtype=np.uint64)),
+        ("dups", rng.integers(0, 7, 100000).astype(np.uint64)),
+        ("equal", np.full(30000, 12345, np.uint64)),
+        ("max", np.full(5000, 2 ** 64 - 1, np.uint64)),
+        ("big", rng.integers(0, 2 ** 64 - 1, 300000, dtype=np.uint64)),
+    ]
+    for name, k in cases:
+        say(f"case sort {name} n={k.size}")
+        v = np.arange(k.size, dtype=np.uint32)[::-1].copy()
+        ko, vo = B.sort_pairs(k, v, 64)
+        o = np.argsort(k, kind="stable")
+        assert np.array_equal(ko, k[o]), name
+        assert np.array_equal(vo, v[o]), name
+    k = rng.integers(0, 2 ** 40, 50000, dtype=np.uint64)
+    ko, vo = B.sort_pairs(k, np.arange(k.size, dtype=np.uint32), 30)
+    low = k & np.uint64(2 ** 30 - 1)
+    o = np.argsort(low, kind="stable")
+    assert np.array_equal(vo, o.astype(np.uint32))
