@@ -52,7 +52,7 @@ class Timings(C.Structure):
         ("leaf_depth", C.c_int32),
         ("max_level", C.c_int32),
         ("walks", C.c_int32),
-        ("reserved", C.c_int32),
+        ("groups", C.c_int32),
     ]
 
 

@@ -131,6 +131,9 @@ bool alloc_nodes(bh_hip_handle* h, size_t cap)
     dfree(h, d.wnode);
     dfree(h, d.scan_tmp);
     dfree(h, d.scan_blocks);
+    dfree(h, d.nparent);
+    dfree(h, d.gscan);
+    dfree(h, d.gscan_blocks);
     d.nprefix = nullptr;
     d.node_cap = cap;
     const size_t scan_n = std::min(d.n, cap);
@@ -138,7 +141,9 @@ bool alloc_nodes(bh_hip_handle* h, size_t cap)
            dalloc(h, &d.ncount, cap, "node count") && dalloc(h, &d.nchild, cap, "node nchild") &&
            dalloc(h, &d.nfirst, cap, "node first") && dalloc(h, &d.nlevel, cap, "node level") &&
            dalloc(h, &d.wnode, cap, "walk nodes") && dalloc(h, &d.scan_tmp, scan_n, "scan tmp") &&
-           dalloc(h, &d.scan_blocks, bh::scan_block_sums_words(scan_n), "scan blocks");
+           dalloc(h, &d.scan_blocks, bh::scan_block_sums_words(scan_n), "scan blocks") &&
+           dalloc(h, &d.nparent, cap, "node parent") && dalloc(h, &d.gscan, cap, "group scan") &&
+           dalloc(h, &d.gscan_blocks, bh::scan_block_sums_words(cap), "group scan blocks");
 }
 
 bool alloc_all(bh_hip_handle* h, uint64_t n)
@@ -156,7 +161,8 @@ bool alloc_all(bh_hip_handle* h, uint64_t n)
         !dalloc(h, &d.rank, n, "rank") || !dalloc(h, &d.keys, n, "keys") ||
         !dalloc(h, &d.keys_b, n, "keys_b") || !dalloc(h, &d.vals, n, "vals") ||
         !dalloc(h, &d.vals_b, n, "vals_b") || !dalloc(h, &d.spart, n, "spart") ||
-        !dalloc(h, &d.sleaf, n, "sleaf") || !dalloc(h, &d.stage, 3 * n, "stage"))
+        !dalloc(h, &d.sleaf, n, "sleaf") || !dalloc(h, &d.stage, 3 * n, "stage") ||
+        !dalloc(h, &d.groups, n, "groups"))
         return false;
     d.sort_hist_cap = bh::radix_hist_words(n);
     if (!dalloc(h, &d.sort_hist, d.sort_hist_cap, "sort hist"))
@@ -242,8 +248,10 @@ void queue_build(bh_hip_handle* h, StepEvents* ev)
     bh::launch_tie_fix(d);
     bh::launch_permute(d);
     bh::launch_tree_build(d, h->leaf_depth, h->p.bucket_capacity);
-    if (h->p.key_mode == 0)
+    if (h->p.key_mode == 0) {
         bh::launch_tree_moments(d);
+        bh::launch_groups(d);
+    }
     rec(ev ? ev->b1 : nullptr, d.stream);
     h->have_tree = true;
 }
@@ -301,10 +309,11 @@ void end_call(bh_hip_handle* h)
         t.pp_interactions = st[0];
         t.pc_interactions = st[1];
     }
-    int info[2 * bh::kLevels + 2];
+    int info[2 * bh::kLevels + 4];
     if (hipMemcpy(info, h->d.level_info, sizeof(info), hipMemcpyDeviceToHost) == hipSuccess) {
-        t.node_count = static_cast<uint64_t>(info[2 * bh::kLevels]);
-        t.max_level = info[2 * bh::kLevels + 1];
+        t.node_count = static_cast<uint64_t>(info[bh::kInfoTotal]);
+        t.max_level = info[bh::kInfoMaxLevel];
+        t.groups = info[bh::kInfoGroups];
     }
     t.leaf_depth = h->leaf_depth;
     t.walks = static_cast<int32_t>(h->events_used);
@@ -586,7 +595,7 @@ bh_hip_result_t bh_hip_tree_info(bh_hip_handle_t* h, uint64_t* node_count, int32
 {
     if (!h)
         return BH_HIP_INVALID_ARGUMENT;
-    int info[2 * bh::kLevels + 2];
+    int info[2 * bh::kLevels + 4];
     if (!check(h, hipStreamSynchronize(h->d.stream), "tree_info") ||
         !check(h, hipMemcpy(info, h->d.level_info, sizeof(info), hipMemcpyDeviceToHost),
                "tree_info"))

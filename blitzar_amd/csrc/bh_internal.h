@@ -11,6 +11,10 @@ namespace bh {
 constexpr int kMaxLevel = 21;          // 21 bits per axis, 63-bit keys
 constexpr int kLevels = kMaxLevel + 1;  // levels 0..21
 constexpr uint32_t kQMax = 2097151u;    // 2^21 - 1
+constexpr uint32_t kGroupMax = 64;      // targets per walk group (one wavefront)
+constexpr int kInfoTotal = 2 * kLevels;      // level_info slot: total nodes
+constexpr int kInfoMaxLevel = 2 * kLevels + 1;  // deepest populated level
+constexpr int kInfoGroups = 2 * kLevels + 2;    // number of walk groups
 
 // Root geometry, written on the device once per build by geometry_kernel.
 //   FP32 part: the quantisation frame of the key kernel (AABB of the
@@ -114,10 +118,15 @@ struct Device {
     uint32_t* nchild = nullptr;   // child count (0 = leaf)
     uint32_t* nfirst = nullptr;   // first child index
     uint8_t* nlevel = nullptr;
+    uint32_t* nparent = nullptr;  // parent index (root: 0xffffffff)
     WalkNode* wnode = nullptr;
+    uint32_t* gscan = nullptr;        // per-node group counts -> offsets
+    uint32_t* gscan_blocks = nullptr;
+    uint4* groups = nullptr;          // walk groups: first particle, count, node
+
     uint32_t* scan_tmp = nullptr;  // per-level scan scratch
     uint32_t* scan_blocks = nullptr;
-    int* level_info = nullptr;  // [0..22) base, [22..44) count, [44] total, [45] max level
+    int* level_info = nullptr;  // [0..22) base, [22..44) count, [44] total, [45] max level, [46] groups
 
     RootGeom* geom = nullptr;
     unsigned long long* aabb_bits = nullptr;  // 6 ordered-encoded doubles
@@ -152,6 +161,10 @@ void launch_permute(Device& d);
 // k_tree.hip
 void launch_tree_build(Device& d, int leaf_depth, int bucket);
 void launch_tree_moments(Device& d);
+// Walk groups: a partition of the sorted particles into spatially compact
+// runs of at most kGroupMax (consecutive small children of one node packed
+// together; see node_groups in k_tree.hip).
+void launch_groups(Device& d);
 // k_walk.hip
 // stats (optional, device): [0] particle-particle, [1] particle-cell interactions
 void launch_walk(Device& d, const WalkParams& p, unsigned long long* stats);

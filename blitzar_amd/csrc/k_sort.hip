@@ -67,7 +67,7 @@ __device__ inline uint32_t block_exclusive256(uint32_t v, uint32_t* tmp, uint32_
 }
 
 __global__ __launch_bounds__(kBlock) void hist_kernel(const unsigned long long* __restrict__ keys,
-                                                      size_t n, int shift,
+                                                      size_t n, int shift, uint32_t dmask,
                                                       uint32_t* __restrict__ hist,
                                                       uint32_t ntiles)
 {
@@ -79,7 +79,7 @@ __global__ __launch_bounds__(kBlock) void hist_kernel(const unsigned long long* 
     for (int k = 0; k < kItems; ++k) {
         const size_t i = base + static_cast<size_t>(k) * kBlock + t;
         if (i < n)
-            atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+            atomicAdd(&h[(keys[i] >> shift) & dmask], 1u);
     }
     __syncthreads();
     hist[static_cast<size_t>(t) * ntiles + blockIdx.x] = h[t];
@@ -88,7 +88,7 @@ __global__ __launch_bounds__(kBlock) void hist_kernel(const unsigned long long* 
 __global__ __launch_bounds__(kBlock) void scatter_kernel(
     const unsigned long long* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     unsigned long long* __restrict__ keys_out, uint32_t* __restrict__ vals_out, size_t n,
-    int shift, const uint32_t* __restrict__ hist, uint32_t ntiles)
+    int shift, uint32_t dmask, const uint32_t* __restrict__ hist, uint32_t ntiles)
 {
     __shared__ uint32_t wcnt[kWaves][256];
     __shared__ uint32_t tile_off[256];
@@ -126,7 +126,7 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(
     for (int it = 0; it < kItems; ++it) {
         const size_t i = wbase + static_cast<size_t>(it) * 64 + lane;
         const bool valid = i < n;
-        const uint32_t d = static_cast<uint32_t>(key[it] >> shift) & 255u;
+        const uint32_t d = static_cast<uint32_t>(key[it] >> shift) & dmask;
         unsigned long long peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(
     for (int it = 0; it < kItems; ++it) {
         const size_t i = wbase + static_cast<size_t>(it) * 64 + lane;
         if (i < n) {
-            const uint32_t d = static_cast<uint32_t>(key[it] >> shift) & 255u;
+            const uint32_t d = static_cast<uint32_t>(key[it] >> shift) & dmask;
             const uint32_t pos = tile_off[d] + wcnt[w][d] + rnk[it];
             sk[pos] = key[it];
             sv[pos] = val[it];
@@ -173,7 +173,7 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(
         const uint32_t j = static_cast<uint32_t>(m) * kBlock + t;
         if (j < tile_n) {
             const unsigned long long k = sk[j];
-            const uint32_t d = static_cast<uint32_t>(k >> shift) & 255u;
+            const uint32_t d = static_cast<uint32_t>(k >> shift) & dmask;
             const size_t out = static_cast<size_t>(gbase[d]) + (j - tile_off[d]);
             keys_out[out] = k;
             vals_out[out] = sv[j];
@@ -371,10 +371,13 @@ void launch_radix_sort(Device& d, size_t n, int key_bits)
     const int passes = (key_bits + 7) / 8;
     for (int p = 0; p < passes; ++p) {
         const int shift = 8 * p;
-        hist_kernel<<<ntiles, kBlock, 0, d.stream>>>(kin, n, shift, hist, ntiles);
+        // The last digit covers only the remaining key bits.
+        const int bits = key_bits - shift < 8 ? key_bits - shift : 8;
+        const uint32_t dmask = (1u << bits) - 1u;
+        hist_kernel<<<ntiles, kBlock, 0, d.stream>>>(kin, n, shift, dmask, hist, ntiles);
         launch_scan_exclusive(hist, hn, nullptr, hn, sums, nullptr, nullptr, d.stream);
-        scatter_kernel<<<ntiles, kBlock, 0, d.stream>>>(kin, vin, kout, vout, n, shift, hist,
-                                                        ntiles);
+        scatter_kernel<<<ntiles, kBlock, 0, d.stream>>>(kin, vin, kout, vout, n, shift, dmask,
+                                                        hist, ntiles);
         std::swap(kin, kout);
         std::swap(vin, vout);
     }

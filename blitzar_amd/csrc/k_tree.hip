@@ -51,8 +51,8 @@ __device__ inline int* level_base(int* info) { return info; }
 __device__ inline int* level_count(int* info) { return info + kLevels; }
 
 __global__ void tree_root_kernel(uint64_t* nprefix, uint8_t* nlevel, uint32_t* nstart,
-                                 uint32_t* ncount, uint32_t* nchild, uint32_t* nfirst, int* info,
-                                 uint32_t n)
+                                 uint32_t* ncount, uint32_t* nchild, uint32_t* nfirst,
+                                 uint32_t* nparent, int* info, uint32_t n)
 {
     const int t = threadIdx.x;
     if (t < kLevels) {
@@ -66,6 +66,7 @@ __global__ void tree_root_kernel(uint64_t* nprefix, uint8_t* nlevel, uint32_t* n
         ncount[0] = n;
         nchild[0] = 0;
         nfirst[0] = 0;
+        nparent[0] = 0xffffffffu;
         info[kLevels] = 1;  // count[0]
         info[2 * kLevels] = 1;
         info[2 * kLevels + 1] = 0;
@@ -155,7 +156,7 @@ __global__ void tree_level_fix_kernel(int* info, int level, int node_cap, int* e
 __global__ __launch_bounds__(256) void tree_emit_kernel(
     const unsigned long long* __restrict__ keys, uint64_t* __restrict__ nprefix,
     uint8_t* __restrict__ nlevel, uint32_t* __restrict__ nstart, uint32_t* __restrict__ ncount,
-    uint32_t* __restrict__ nchild, uint32_t* __restrict__ nfirst,
+    uint32_t* __restrict__ nchild, uint32_t* __restrict__ nfirst, uint32_t* __restrict__ nparent,
     const uint32_t* __restrict__ scan_tmp, const int* __restrict__ info, int level)
 {
     const int base = info[level];
@@ -189,6 +190,7 @@ __global__ __launch_bounds__(256) void tree_emit_kernel(
             ncount[child] = bnd[o + 1] - bnd[o];
             nchild[child] = 0;
             nfirst[child] = bnd[o];
+            nparent[child] = static_cast<uint32_t>(node);
         }
     }
 }
@@ -313,6 +315,89 @@ __global__ __launch_bounds__(256) void tree_moments_kernel(
     }
 }
 
+
+// Walk groups. Node i owns groups when
+//   * i is the root and holds <= G particles: one group;
+//   * i holds > G particles and is a leaf: ceil(count / G) chunks;
+//   * i holds > G particles and has children: its children holding <= G
+//     particles, packed greedily in octant order into runs of consecutive
+//     children with <= G particles in total (a child holding > G ends a run).
+// Every particle lands in exactly one group; a group's particles are
+// contiguous in sorted order and lie inside node i's cell, whose lattice
+// centre is the group's reference point. G = kGroupMax.
+template <bool EMIT>
+__device__ inline uint32_t node_groups(int i, const uint32_t* __restrict__ nstart,
+                                       const uint32_t* __restrict__ ncount,
+                                       const uint32_t* __restrict__ nchild,
+                                       const uint32_t* __restrict__ nfirst,
+                                       const uint32_t* __restrict__ nparent, uint32_t base,
+                                       uint4* __restrict__ groups)
+{
+    const uint32_t c = ncount[i];
+    if (c <= kGroupMax) {
+        if (nparent[i] != 0xffffffffu)
+            return 0;
+        if (EMIT)
+            groups[base] = make_uint4(nstart[i], c, static_cast<uint32_t>(i), 0u);
+        return 1;
+    }
+    const uint32_t nc = nchild[i];
+    if (nc == 0) {
+        const uint32_t k = (c + kGroupMax - 1) / kGroupMax;
+        if (EMIT)
+            for (uint32_t j = 0; j < k; ++j) {
+                const uint32_t off = j * kGroupMax;
+                const uint32_t cnt = c - off < kGroupMax ? c - off : kGroupMax;
+                groups[base + j] = make_uint4(nstart[i] + off, cnt, static_cast<uint32_t>(i), 0u);
+            }
+        return k;
+    }
+    uint32_t ng = 0, run_start = 0, run_count = 0;
+    const uint32_t f = nfirst[i];
+    for (uint32_t j = f; j < f + nc; ++j) {
+        const uint32_t cc = ncount[j];
+        const bool small = cc <= kGroupMax;
+        if (run_count > 0 && (!small || run_count + cc > kGroupMax)) {
+            if (EMIT)
+                groups[base + ng] = make_uint4(run_start, run_count, static_cast<uint32_t>(i), 0u);
+            ++ng;
+            run_count = 0;
+        }
+        if (small) {
+            if (run_count == 0)
+                run_start = nstart[j];
+            run_count += cc;
+        }
+    }
+    if (run_count > 0) {
+        if (EMIT)
+            groups[base + ng] = make_uint4(run_start, run_count, static_cast<uint32_t>(i), 0u);
+        ++ng;
+    }
+    return ng;
+}
+
+__global__ __launch_bounds__(256) void group_count_kernel(
+    const uint32_t* __restrict__ nstart, const uint32_t* __restrict__ ncount,
+    const uint32_t* __restrict__ nchild, const uint32_t* __restrict__ nfirst,
+    const uint32_t* __restrict__ nparent, const int* __restrict__ info, uint32_t* __restrict__ gcnt)
+{
+    const int total = info[kInfoTotal];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x)
+        gcnt[i] = node_groups<false>(i, nstart, ncount, nchild, nfirst, nparent, 0, nullptr);
+}
+
+__global__ __launch_bounds__(256) void group_emit_kernel(
+    const uint32_t* __restrict__ nstart, const uint32_t* __restrict__ ncount,
+    const uint32_t* __restrict__ nchild, const uint32_t* __restrict__ nfirst,
+    const uint32_t* __restrict__ nparent, const uint32_t* __restrict__ goff,
+    const int* __restrict__ info, uint4* __restrict__ groups)
+{
+    const int total = info[kInfoTotal];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x)
+        node_groups<true>(i, nstart, ncount, nchild, nfirst, nparent, goff[i], groups);
+}
+
 inline unsigned level_grid(size_t bound)
 {
     const size_t b = (bound + 255) / 256;
@@ -333,7 +418,7 @@ void launch_tree_build(Device& d, int leaf_depth, int bucket)
 {
     const uint32_t n = static_cast<uint32_t>(d.n);
     tree_root_kernel<<<1, 64, 0, d.stream>>>(d.nprefix, d.nlevel, d.nstart, d.ncount, d.nchild,
-                                             d.nfirst, d.level_info, n);
+                                             d.nfirst, d.nparent, d.level_info, n);
     const unsigned long long* keys = reinterpret_cast<const unsigned long long*>(d.keys);
     const int cap = static_cast<int>(d.node_cap);
     for (int L = 0; L < kMaxLevel; ++L) {
@@ -346,9 +431,20 @@ void launch_tree_build(Device& d, int leaf_depth, int bucket)
                               nullptr, d.level_info + kLevels + L + 1, d.stream);
         tree_level_fix_kernel<<<1, 64, 0, d.stream>>>(d.level_info, L, cap, d.err);
         tree_emit_kernel<<<g, 256, 0, d.stream>>>(keys, d.nprefix, d.nlevel, d.nstart, d.ncount,
-                                                  d.nchild, d.nfirst, d.scan_tmp, d.level_info,
-                                                  L);
+                                                  d.nchild, d.nfirst, d.nparent, d.scan_tmp,
+                                                  d.level_info, L);
     }
+}
+
+void launch_groups(Device& d)
+{
+    const unsigned g = level_grid(d.node_cap);
+    group_count_kernel<<<g, 256, 0, d.stream>>>(d.nstart, d.ncount, d.nchild, d.nfirst,
+                                                d.nparent, d.level_info, d.gscan);
+    launch_scan_exclusive(d.gscan, d.node_cap, d.level_info + kInfoTotal, 0, d.gscan_blocks,
+                          nullptr, d.level_info + kInfoGroups, d.stream);
+    group_emit_kernel<<<g, 256, 0, d.stream>>>(d.nstart, d.ncount, d.nchild, d.nfirst,
+                                               d.nparent, d.gscan, d.level_info, d.groups);
 }
 
 void launch_tree_moments(Device& d)

@@ -10,9 +10,10 @@
 // (pair force engine/physics/octree/force/OctreeForce.cpp:23-57); the sum is
 // clamped to max_acceleration (engine/physics/octree/force/OctForces.cpp:128-133).
 //
-// MI355X mapping. One 64-lane workgroup (one wavefront) owns a group of 64
-// Morton-contiguous targets (one target per lane) and walks the tree once for
-// the whole group. A node is accepted only when the criterion holds for every
+// MI355X mapping. One 64-lane workgroup (one wavefront) owns a group of at
+// most 64 targets (one target per lane) -- the particles of one tree node, so
+// the group is spatially compact -- and walks the tree once for the whole
+// group. A node is accepted only when the criterion holds for every
 // point of the group's bounding box (distance measured from the box, which is
 // conservative), and only when no target lies within the node's bounding
 // sphere (bmax) -- the group form of the reference's "contains self" test.
@@ -23,7 +24,7 @@
 // exits when the ticket passes the group count.
 //
 // Precision: positions in the walk are FP32 offsets from the exact lattice
-// centre of the group's first leaf: float(c_node - c_ref) * unit + offset,
+// centre of the group's node: float(c_node - c_ref) * unit + offset,
 // with integer lattice coordinates c, so the walk itself uses no FP64.
 //
 // Cost model for the roofline: 20 FLOP per particle-particle and 50 FLOP per
@@ -136,7 +137,8 @@ __device__ inline void eval_parts(const float4* __restrict__ pl, int np, float t
 template <int ORDER, int CRIT, bool TPM>
 __global__ __launch_bounds__(64) void walk_kernel(
     WalkParams P, const WalkNode* __restrict__ nodes, const float4* __restrict__ spart,
-    const uint32_t* __restrict__ sleaf, const RootGeom* __restrict__ geom, int n, int ngroups,
+    const uint32_t* __restrict__ sleaf, const RootGeom* __restrict__ geom,
+    const uint4* __restrict__ groups, const int* __restrict__ info,
     int node_cap, unsigned int* __restrict__ work, int* __restrict__ stack_all, int stack_cap,
     float* __restrict__ accx, float* __restrict__ accy, float* __restrict__ accz,
     int* __restrict__ err, unsigned long long* __restrict__ stats)
@@ -156,6 +158,7 @@ __global__ __launch_bounds__(64) void walk_kernel(
     const float wfx = geom->width_f[0], wfy = geom->width_f[1], wfz = geom->width_f[2];
     const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     unsigned long long npp = 0, npc = 0;
+    const int ngroups = info[kInfoGroups];
 
     for (;;) {
         if (lane == 0)
@@ -166,11 +169,12 @@ __global__ __launch_bounds__(64) void walk_kernel(
         if (g >= ngroups)
             break;
 
-        const int s = g * 64 + lane;
-        const bool active = s < n;
-        const int nact = min(64, n - g * 64);
-        const int refLeaf = static_cast<int>(sleaf[g * 64]);
-        const int rcx = nodes[refLeaf].cx, rcy = nodes[refLeaf].cy, rcz = nodes[refLeaf].cz;
+        const uint4 gr = groups[g];
+        const int nact = static_cast<int>(gr.y);
+        const int s = static_cast<int>(gr.x) + lane;
+        const bool active = lane < nact;
+        const int gnode = static_cast<int>(gr.z);
+        const int rcx = nodes[gnode].cx, rcy = nodes[gnode].cy, rcz = nodes[gnode].cz;
 
         float tx = 0.0f, ty = 0.0f, tz = 0.0f;
         if (active) {
@@ -371,9 +375,8 @@ __global__ __launch_bounds__(64) void walk_kernel(
 template <int ORDER, int CRIT, bool TPM>
 void launch_walk_t(Device& d, const WalkParams& p, unsigned grid, unsigned long long* stats)
 {
-    const int ngroups = static_cast<int>((d.n + 63) / 64);
     walk_kernel<ORDER, CRIT, TPM><<<grid, 64, 0, d.stream>>>(
-        p, d.wnode, d.spart, d.sleaf, d.geom, static_cast<int>(d.n), ngroups,
+        p, d.wnode, d.spart, d.sleaf, d.geom, d.groups, d.level_info,
         static_cast<int>(d.node_cap), d.work_counter,
         d.stack, static_cast<int>(d.stack_cap), d.acc[0], d.acc[1], d.acc[2], d.err, stats);
 }
@@ -385,8 +388,7 @@ void launch_walk(Device& d, const WalkParams& p, unsigned long long* stats)
     if (d.n == 0)
         return;
     launch_fill_u32(d.work_counter, 0u, 1, d.stream);
-    const size_t ngroups = (d.n + 63) / 64;
-    const unsigned grid = static_cast<unsigned>(std::min(ngroups, d.stack_waves));
+    const unsigned grid = static_cast<unsigned>(d.stack_waves);
     const bool tpm = p.rs > 0.0f;
     if (p.order >= 2) {
         if (p.criterion == 1)

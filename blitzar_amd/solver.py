@@ -191,7 +191,7 @@ class OctreeHip:
     def timings(self) -> dict:
         t = Timings()
         _check(lib().bh_hip_get_timings(self._h, C.byref(t)), self._h, "get_timings")
-        return {k: getattr(t, k) for k, _ in Timings._fields_ if k != "reserved"}
+        return {k: getattr(t, k) for k, _ in Timings._fields_}
 
 
 def sort_pairs(keys, vals, key_bits: int = 64):

@@ -88,7 +88,7 @@ typedef struct bh_hip_timings {
     int32_t leaf_depth;       /* reference leaf depth of the last tree */
     int32_t max_level;        /* deepest populated level of the last tree */
     int32_t walks;            /* force evaluations the sums above cover */
-    int32_t reserved;
+    int32_t groups;           /* walk groups (target sets) of the last tree */
 } bh_hip_timings_t;
 
 /* Fill `out` with the defaults (reference config defaults, CfgConfig.hpp). */

@@ -110,6 +110,11 @@ def ref_lib():
         R.ref_octree_forces.argtypes = [C.c_int, _f32p, C.c_float, C.c_float, C.c_float,
                                         C.c_float, C.c_float, C.c_int, C.c_float, _f32p]
         R.ref_octree_forces.restype = C.c_int
+        R.ref_octree_forces_subset.argtypes = [C.c_int, _f32p, C.c_float, C.c_float, C.c_float,
+                                               C.c_float, C.c_float, C.c_int, C.c_int, _i32p,
+                                               _f32p, C.c_int, C.POINTER(C.c_double),
+                                               C.POINTER(C.c_double)]
+        R.ref_octree_forces_subset.restype = C.c_int
         _ref = R
     return _ref
 
@@ -309,3 +314,20 @@ def multipole_accel(m10, x3, eps, order=2):
     lib().dx_multipole_accel(np.ascontiguousarray(m10, np.float64),
                              np.ascontiguousarray(x3, np.float64), eps, order, out)
     return out
+
+
+def ref_octree_forces_subset(xyzm, targets, params=None, nthreads=0):
+    """Reference octree (oracle/_ref) on listed targets, OpenMP over targets.
+    Returns (acc, build_seconds, walk_seconds) or None when _ref is absent."""
+    R = ref_lib()
+    if R is None:
+        return None
+    p = params or make_params()
+    t = np.ascontiguousarray(targets, np.int32)
+    acc = np.zeros((t.size, 3), np.float32)
+    tb, tw = C.c_double(), C.c_double()
+    R.ref_octree_forces_subset(xyzm.shape[0], np.ascontiguousarray(xyzm, np.float32).reshape(-1),
+                               p.theta, p.softening, p.min_softening, p.min_distance2,
+                               p.min_theta, p.criterion, t.size, t, acc.reshape(-1), nthreads,
+                               C.byref(tb), C.byref(tw))
+    return acc, tb.value, tw.value
