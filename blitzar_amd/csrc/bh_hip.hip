@@ -34,6 +34,7 @@ struct bh_hip_handle {
     int cu_count = 256;
     bool primed = false;      // acc holds a(x) of the current positions
     bool have_tree = false;   // the last build matches the current positions
+    bool rank_valid = true;   // rank (caller index -> slot) matches the store order
     bool timing = false;
     std::string last_error;
     unsigned long long* stats = nullptr;  // device: pp, pc
@@ -172,7 +173,7 @@ bool alloc_all(bh_hip_handle* h, uint64_t n)
         !dalloc(h, &d.err, 4, "error flags") || !dalloc(h, &d.work_counter, 4, "work counter") ||
         !dalloc(h, &h->stats, 2, "walk stats"))
         return false;
-    d.diag_cap = 8 * 256 + 16;
+    d.diag_cap = 6 * 1024 + 8 * 256 + 16;  // AABB partials, energy partials
     if (!dalloc(h, &d.diag, d.diag_cap, "diagnostics"))
         return false;
     if (!alloc_nodes(h, 2 * n + 4096))
@@ -247,6 +248,7 @@ void queue_build(bh_hip_handle* h, StepEvents* ev)
     rec(ev ? ev->s1 : nullptr, d.stream);
     bh::launch_tie_fix(d);
     bh::launch_permute(d);
+    h->rank_valid = false;
     bh::launch_tree_build(d, h->leaf_depth, h->p.bucket_capacity);
     if (h->p.key_mode == 0) {
         bh::launch_tree_moments(d);
@@ -254,6 +256,14 @@ void queue_build(bh_hip_handle* h, StepEvents* ev)
     }
     rec(ev ? ev->b1 : nullptr, d.stream);
     h->have_tree = true;
+}
+
+void ensure_rank(bh_hip_handle* h)
+{
+    if (!h->rank_valid) {
+        bh::launch_rank(h->d);
+        h->rank_valid = true;
+    }
 }
 
 bh::WalkParams walk_params(const bh_hip_params_t& p)
@@ -441,6 +451,7 @@ bh_hip_result_t bh_hip_upload(bh_hip_handle_t* h, const double* pos3, const doub
     bh::Device& d = h->d;
     // The store keeps its current order; rank maps caller index -> slot.
     std::vector<float> m_sorted;
+    ensure_rank(h);
     bool ok = zero_err(h) && stage_to_device(h, pos3);
     if (ok) {
         bh::launch_upload_scatter(d, 0);
@@ -484,6 +495,7 @@ bh_hip_result_t bh_hip_download(bh_hip_handle_t* h, double* pos3, double* vel3)
     if (!h)
         return BH_HIP_INVALID_ARGUMENT;
     bh::Device& d = h->d;
+    ensure_rank(h);
     for (int which = 0; which < 2; ++which) {
         double* dst = which == 0 ? pos3 : vel3;
         if (!dst)
@@ -528,6 +540,7 @@ bh_hip_result_t bh_hip_compute_accelerations(bh_hip_handle_t* h, float* acc3)
     if (r != BH_HIP_OK || !acc3)
         return r;
     bh::Device& d = h->d;
+    ensure_rank(h);
     bh::launch_download_gather(d, 2);
     std::vector<double> tmp(3 * d.n);
     if (!check(h, hipMemcpyAsync(tmp.data(), d.stage, tmp.size() * sizeof(double),
@@ -681,6 +694,7 @@ bh_hip_result_t bh_hip_direct_forces_fp64(bh_hip_handle_t* h, const uint32_t* ta
     bool ok = dalloc(h, &dt, nt, "targets") && dalloc(h, &dout, 3 * static_cast<size_t>(nt), "out");
     ok = ok && check(h, hipMemcpy(dt, targets, nt * 4, hipMemcpyHostToDevice), "targets");
     if (ok) {
+        ensure_rank(h);
         bh::launch_direct_fp64(h->d, dt, nt, h->p.softening, h->p.min_distance2, dout);
         ok = check(h, hipGetLastError(), "direct launch") &&
              check(h, hipStreamSynchronize(h->d.stream), "direct") &&

@@ -158,6 +158,7 @@ void launch_scan_exclusive(uint32_t* data, size_t max_n, const int* n_dev, size_
 size_t scan_block_sums_words(size_t max_n);
 void launch_tie_fix(Device& d);
 void launch_permute(Device& d);
+void launch_rank(Device& d);  // rebuild rank from id
 // k_tree.hip
 void launch_tree_build(Device& d, int leaf_depth, int bucket);
 void launch_tree_moments(Device& d);

@@ -12,7 +12,8 @@
 // with the 10-bit masks and therefore keeps only the low 10 bits per axis.
 //
 // HBM traffic: AABB reads 24 B/particle (FP64 SoA); keys read 24 B and write
-// 12 B per particle.
+// 12 B per particle. The AABB combines per-block partial extrema in a second
+// one-block kernel (uses Device::diag, 6 doubles per block, 1024 blocks).
 #include "bh_internal.h"
 
 #include <cfloat>
@@ -34,14 +35,7 @@ __device__ inline double from_order_bits(unsigned long long e)
     return __longlong_as_double(static_cast<long long>(b));
 }
 
-__global__ void aabb_init_kernel(unsigned long long* bits)
-{
-    const int t = threadIdx.x;
-    if (t < 3)
-        bits[t] = ~0ull;  // min slots
-    else if (t < 6)
-        bits[t] = 0ull;   // max slots
-}
+constexpr int kAabbBlocks = 1024;
 
 __device__ inline double wave_min(double v)
 {
@@ -57,14 +51,15 @@ __device__ inline double wave_max(double v)
     return v;
 }
 
-// Grid-stride min/max of the three FP64 coordinate arrays; one atomic per
-// wave and slot on order-preserving integer encodings (order independent, so
-// the result is deterministic).
+// Grid-stride min/max of the three FP64 coordinate arrays; each block writes
+// its six partial extrema (no atomics; the final kernel combines them, so the
+// result is deterministic).
 __global__ __launch_bounds__(256) void aabb_kernel(const double* __restrict__ px,
                                                    const double* __restrict__ py,
                                                    const double* __restrict__ pz, size_t n,
-                                                   unsigned long long* bits)
+                                                   double* __restrict__ partial)
 {
+    __shared__ double red[6][4];
     double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
     double hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
     const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
@@ -78,15 +73,53 @@ __global__ __launch_bounds__(256) void aabb_kernel(const double* __restrict__ px
         hi[1] = fmax(hi[1], y);
         hi[2] = fmax(hi[2], z);
     }
+    const int w = threadIdx.x >> 6;
     for (int a = 0; a < 3; ++a) {
-        lo[a] = wave_min(lo[a]);
-        hi[a] = wave_max(hi[a]);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        for (int a = 0; a < 3; ++a) {
-            atomicMin(&bits[a], order_bits(lo[a]));
-            atomicMax(&bits[3 + a], order_bits(hi[a]));
+        const double l = wave_min(lo[a]);
+        const double h = wave_max(hi[a]);
+        if ((threadIdx.x & 63) == 0) {
+            red[a][w] = l;
+            red[3 + a][w] = h;
         }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int a = threadIdx.x;
+        double v = red[a][0];
+        for (int k = 1; k < 4; ++k)
+            v = a < 3 ? fmin(v, red[a][k]) : fmax(v, red[a][k]);
+        partial[6 * blockIdx.x + a] = v;
+    }
+}
+
+// Combine the per-block extrema (one block).
+__global__ __launch_bounds__(256) void aabb_final_kernel(const double* __restrict__ partial,
+                                                         int nb, unsigned long long* bits)
+{
+    __shared__ double red[6][4];
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+    double hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fmin(lo[a], partial[6 * b + a]);
+            hi[a] = fmax(hi[a], partial[6 * b + 3 + a]);
+        }
+    const int w = threadIdx.x >> 6;
+    for (int a = 0; a < 3; ++a) {
+        const double l = wave_min(lo[a]);
+        const double h = wave_max(hi[a]);
+        if ((threadIdx.x & 63) == 0) {
+            red[a][w] = l;
+            red[3 + a][w] = h;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int a = threadIdx.x;
+        double v = red[a][0];
+        for (int k = 1; k < 4; ++k)
+            v = a < 3 ? fmin(v, red[a][k]) : fmax(v, red[a][k]);
+        bits[a] = order_bits(v);
     }
 }
 
@@ -166,10 +199,10 @@ __global__ __launch_bounds__(256) void keys_kernel(const double* __restrict__ px
 
 void launch_aabb(Device& d)
 {
-    aabb_init_kernel<<<1, 64, 0, d.stream>>>(d.aabb_bits);
-    const size_t blocks = std::min<size_t>((d.n + 255) / 256, 2048);
-    aabb_kernel<<<static_cast<unsigned>(std::max<size_t>(blocks, 1)), 256, 0, d.stream>>>(
-        d.pos[0], d.pos[1], d.pos[2], d.n, d.aabb_bits);
+    const size_t blocks = std::min<size_t>((d.n + 255) / 256, kAabbBlocks);
+    const int nb = static_cast<int>(std::max<size_t>(blocks, 1));
+    aabb_kernel<<<nb, 256, 0, d.stream>>>(d.pos[0], d.pos[1], d.pos[2], d.n, d.diag);
+    aabb_final_kernel<<<1, 256, 0, d.stream>>>(d.diag, nb, d.aabb_bits);
     geometry_kernel<<<1, 64, 0, d.stream>>>(d.aabb_bits, d.geom);
 }
 

@@ -309,8 +309,7 @@ __global__ __launch_bounds__(256) void permute_kernel(
     const double* __restrict__ vy, const double* __restrict__ vz, const float* __restrict__ m,
     const uint32_t* __restrict__ id, double* __restrict__ opx, double* __restrict__ opy,
     double* __restrict__ opz, double* __restrict__ ovx, double* __restrict__ ovy,
-    double* __restrict__ ovz, float* __restrict__ om, uint32_t* __restrict__ oid,
-    uint32_t* __restrict__ rank)
+    double* __restrict__ ovz, float* __restrict__ om, uint32_t* __restrict__ oid)
 {
     const size_t s = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (s >= n)
@@ -323,9 +322,17 @@ __global__ __launch_bounds__(256) void permute_kernel(
     ovy[s] = vy[j];
     ovz[s] = vz[j];
     om[s] = m[j];
-    const uint32_t pid = id[j];
-    oid[s] = pid;
-    rank[pid] = static_cast<uint32_t>(s);
+    oid[s] = id[j];
+}
+
+// rank[id[s]] = s: caller index -> store slot (rebuilt only when an API call
+// needs it, not every step).
+__global__ __launch_bounds__(256) void rank_kernel(const uint32_t* __restrict__ id, size_t n,
+                                                   uint32_t* __restrict__ rank)
+{
+    const size_t s = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s < n)
+        rank[id[s]] = static_cast<uint32_t>(s);
 }
 
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, size_t n)
@@ -404,14 +411,20 @@ void launch_permute(Device& d)
     const unsigned blocks = static_cast<unsigned>((d.n + 255) / 256);
     permute_kernel<<<blocks, 256, 0, d.stream>>>(
         d.vals, d.n, d.pos[0], d.pos[1], d.pos[2], d.vel[0], d.vel[1], d.vel[2], d.mass, d.id,
-        d.pos_b[0], d.pos_b[1], d.pos_b[2], d.vel_b[0], d.vel_b[1], d.vel_b[2], d.mass_b, d.id_b,
-        d.rank);
+        d.pos_b[0], d.pos_b[1], d.pos_b[2], d.vel_b[0], d.vel_b[1], d.vel_b[2], d.mass_b, d.id_b);
     for (int a = 0; a < 3; ++a) {
         std::swap(d.pos[a], d.pos_b[a]);
         std::swap(d.vel[a], d.vel_b[a]);
     }
     std::swap(d.mass, d.mass_b);
     std::swap(d.id, d.id_b);
+}
+
+void launch_rank(Device& d)
+{
+    if (d.n == 0)
+        return;
+    rank_kernel<<<static_cast<unsigned>((d.n + 255) / 256), 256, 0, d.stream>>>(d.id, d.n, d.rank);
 }
 
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s)
