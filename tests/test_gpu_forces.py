@@ -86,7 +86,9 @@ def test_small_theta_is_direct_sum(oracle, order):
     assert np.median(r) < 1e-5
     assert np.percentile(r, 99) < (1e-3 if order == 1 else 1e-4)
     rms = np.sqrt((ref ** 2).sum(1).mean())
-    assert np.linalg.norm(a[t] - ref, axis=1).max() / rms < (1e-3 if order == 1 else 1e-4)
+    # an outskirts target sees the whole core as one accepted cell: the
+    # neglected quadrupole term is ~(theta/2)^2 of the force for the monopole
+    assert np.linalg.norm(a[t] - ref, axis=1).max() / rms < (5e-3 if order == 1 else 1e-4)
 
 
 @pytest.mark.parametrize("criterion", ["com", "bounds"])
