@@ -83,6 +83,44 @@ void dx_force_error(int nt, const double* approx3, const double* ref3, double* o
     out[4] = rel.back();
 }
 
+// FP64 direct sum of the TreePM short-range force: the Newtonian pair force
+// times w(r) = erfc(r / 2rs) + r / (rs sqrt(pi)) exp(-r^2 / 4rs^2), with r the
+// softened distance (engine/physics/octree/force/OctreeForce.cpp:49-57), and
+// pairs beyond the cutoff rc (unsoftened distance) dropped
+// (OctreeTraversal.cpp:172-176).
+void dx_direct_tpm_fp64(int n, const double* pos3, const double* mass, int nt, const int* targets,
+                        double eps, double rs, double rc, double* acc3, int nthreads)
+{
+    const int threads = nthreads > 0 ? nthreads : omp_get_max_threads();
+    const double e2 = eps * eps;
+    const double rc2 = rc * rc;
+    const double isp = 0.5641895835477563;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(threads)
+    for (int t = 0; t < nt; ++t) {
+        const int i = targets[t];
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        for (int j = 0; j < n; ++j) {
+            if (j == i)
+                continue;
+            const double dx = pos3[3 * j] - pos3[3 * i], dy = pos3[3 * j + 1] - pos3[3 * i + 1],
+                         dz = pos3[3 * j + 2] - pos3[3 * i + 2];
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            if (rc > 0.0 && d2 > rc2)
+                continue;
+            const double r = std::sqrt(d2 + e2);
+            const double a = 0.5 * r / rs;
+            const double w = std::erfc(a) + r * isp / rs * std::exp(-a * a);
+            const double s = mass[j] * w / (r * r * r);
+            ax += dx * s;
+            ay += dy * s;
+            az += dz * s;
+        }
+        acc3[3 * t] = ax;
+        acc3[3 * t + 1] = ay;
+        acc3[3 * t + 2] = az;
+    }
+}
+
 // Kinetic energy, softened pair potential (i<j), linear and angular momentum.
 void dx_energy(int n, const double* pos3, const double* vel3, const double* mass, double eps,
                double min_d2, double* out8, int nthreads)

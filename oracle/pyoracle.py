@@ -94,6 +94,8 @@ def lib():
         L.dx_direct_fp64.argtypes = [C.c_int, _f64p, _f64p, C.c_int, _i32p, C.c_double,
                                      C.c_double, _f64p, C.c_int]
         L.dx_force_error.argtypes = [C.c_int, _f64p, _f64p, _f64p]
+        L.dx_direct_tpm_fp64.argtypes = [C.c_int, _f64p, _f64p, C.c_int, _i32p, C.c_double,
+                                         C.c_double, C.c_double, _f64p, C.c_int]
         L.dx_energy.argtypes = [C.c_int, _f64p, _f64p, _f64p, C.c_double, C.c_double, _f64p,
                                 C.c_int]
         L.dx_moments.argtypes = [C.c_int, _f64p, _f64p, _f64p]
@@ -283,6 +285,15 @@ def direct_fp64(pos, mass, targets, eps, min_d2=0.0, nthreads=0):
     t = np.ascontiguousarray(targets, np.int32)
     acc = np.zeros((t.size, 3), np.float64)
     lib().dx_direct_fp64(m.size, p, m, t.size, t, eps, min_d2, acc.reshape(-1), nthreads)
+    return acc
+
+
+def direct_tpm_fp64(pos, mass, targets, eps, rs, rc, nthreads=0):
+    p = np.ascontiguousarray(pos, np.float64).reshape(-1)
+    m = np.ascontiguousarray(mass, np.float64)
+    t = np.ascontiguousarray(targets, np.int32)
+    acc = np.zeros((t.size, 3), np.float64)
+    lib().dx_direct_tpm_fp64(m.size, p, m, t.size, t, eps, rs, rc, acc.reshape(-1), nthreads)
     return acc
 
 
