@@ -56,6 +56,22 @@ class Timings(C.Structure):
     ]
 
 
+class LffConfig(C.Structure):
+    _fields_ = [
+        ("theta", C.c_float),
+        ("band", C.c_float),
+        ("list_cap", C.c_int32),
+        ("enabled", C.c_int32),
+        ("min_count", C.c_int32),
+    ]
+
+
+LFF_INPUTS = 16
+LFF_HIDDEN = 64
+LFF_OUTPUTS = 3
+LFF_PARAMS = 5443
+
+
 # Every exported symbol of include/bh_hip.h and include/bh_ic.h.
 EXPORTS = [
     "bh_hip_default_params", "bh_hip_version", "bh_hip_create", "bh_hip_destroy",
@@ -63,6 +79,9 @@ EXPORTS = [
     "bh_hip_step", "bh_hip_build_tree", "bh_hip_tree_info", "bh_hip_dump_tree",
     "bh_hip_direct_forces_fp64", "bh_hip_diagnostics", "bh_hip_set_timing",
     "bh_hip_get_timings", "bh_hip_get_last_error", "bh_hip_sort_pairs",
+    "bh_hip_lff_configure", "bh_hip_lff_collect", "bh_hip_lff_train", "bh_hip_lff_get_params",
+    "bh_hip_lff_set_params", "bh_hip_lff_gradient", "bh_hip_lff_forward",
+    "bh_hip_lff_get_tuples",
     "bh_ic_plummer", "bh_ic_hernquist", "bh_ic_hernquist_merger", "bh_ic_uniform_cube",
 ]
 
@@ -139,6 +158,21 @@ def lib() -> C.CDLL:
     L.bh_hip_sort_pairs.argtypes = [_p(np.uint64), _p(np.uint32), C.c_uint64, C.c_int32,
                                     _p(np.uint64), _p(np.uint32), C.c_char_p, C.c_size_t]
     L.bh_hip_sort_pairs.restype = R
+    L.bh_hip_lff_configure.argtypes = [H, C.POINTER(LffConfig)]
+    L.bh_hip_lff_collect.argtypes = [H, C.c_int32, C.c_uint64, C.POINTER(C.c_uint64)]
+    L.bh_hip_lff_train.argtypes = [H, C.c_int32, C.c_int32, C.c_float, C.c_uint64,
+                                   _opt(np.float32)]
+    L.bh_hip_lff_get_params.argtypes = [H, _p(np.float32)]
+    L.bh_hip_lff_set_params.argtypes = [H, _p(np.float32)]
+    L.bh_hip_lff_gradient.argtypes = [H, _p(np.float32), _p(np.float32), C.c_uint32,
+                                      _p(np.float32), _p(np.float32), C.POINTER(C.c_float)]
+    L.bh_hip_lff_forward.argtypes = [H, _p(np.float32), C.c_uint32, _p(np.float32)]
+    L.bh_hip_lff_get_tuples.argtypes = [H, _opt(np.float32), _opt(np.float32), C.c_uint64,
+                                        C.POINTER(C.c_uint64)]
+    for name in ("bh_hip_lff_configure", "bh_hip_lff_collect", "bh_hip_lff_train",
+                 "bh_hip_lff_get_params", "bh_hip_lff_set_params", "bh_hip_lff_gradient",
+                 "bh_hip_lff_forward", "bh_hip_lff_get_tuples"):
+        getattr(L, name).restype = R
     L.bh_ic_plummer.argtypes = [C.c_uint64, C.c_uint64, C.c_double, C.c_double, _p(np.float64),
                                 _p(np.float64), _p(np.float32)]
     L.bh_ic_plummer.restype = None

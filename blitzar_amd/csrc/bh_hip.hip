@@ -13,6 +13,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <numeric>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -44,6 +46,12 @@ struct bh_hip_handle {
     hipEvent_t i0 = nullptr, i1 = nullptr;
     bh_hip_timings_t t{};
     std::vector<void*> allocations;
+    // learned far-field correction
+    bh::LffState lff;
+    bh_hip_lff_config_t lff_cfg{0.9f, 0.5f, 192, 0, 32};
+    float* lff_train_params = nullptr;  // normalised-output copy used while training
+    uint32_t* lff_order = nullptr;
+    size_t lff_order_cap = 0;
 };
 
 namespace {
@@ -280,12 +288,37 @@ bh::WalkParams walk_params(const bh_hip_params_t& p)
     return w;
 }
 
-// Queue build + walk (a force evaluation).
+bool lff_ready(bh_hip_handle* h) { return h->lff_cfg.enabled && h->lff.lists && h->lff.frag; }
+
+// The corrected walk exists for the quadrupole, non-periodic, 63-bit-key path.
+const char* lff_incompatible(const bh_hip_params_t& p)
+{
+    if (p.multipole_order != 2)
+        return "learned far-field correction needs multipole_order = 2";
+    if (p.treepm_split_scale > 0.0f)
+        return "learned far-field correction is not available with the TreePM short-range walk";
+    if (p.key_mode != 0)
+        return "learned far-field correction needs the 63-bit keys";
+    return nullptr;
+}
+
+// Queue build + walk (a force evaluation); with the learned correction
+// enabled the walk runs at the correction's opening angle, records the near
+// nodes of each group and the MLP pass adds its corrections.
 void queue_force(bh_hip_handle* h)
 {
     StepEvents* ev = next_events(h);
     queue_build(h, ev);
-    bh::launch_walk(h->d, walk_params(h->p), h->stats);
+    if (lff_ready(h)) {
+        bh::WalkParams wp = walk_params(h->p);
+        wp.theta2 = h->lff_cfg.theta * h->lff_cfg.theta;
+        wp.max_acc = 0.0f;  // clamped after the correction
+        bh::launch_walk_record(h->d, wp, h->stats, h->lff, h->lff_cfg.band, h->lff_cfg.min_count);
+        bh::launch_lff_apply(h->d, h->lff, h->lff_cfg.band, wp.eps2, h->lff_cfg.min_count,
+                             h->p.max_acceleration);
+    } else {
+        bh::launch_walk(h->d, walk_params(h->p), h->stats);
+    }
     rec(ev ? ev->w1 : nullptr, h->d.stream);
     h->primed = true;
 }
@@ -435,7 +468,14 @@ bh_hip_result_t bh_hip_set_params(bh_hip_handle_t* h, const bh_hip_params_t* par
     if (!h || !params)
         return BH_HIP_INVALID_ARGUMENT;
     const bh_hip_params_t old = h->p;
-    h->p = resolve(*params);
+    const bh_hip_params_t next = resolve(*params);
+    if (h->lff_cfg.enabled) {
+        if (const char* why = lff_incompatible(next)) {
+            h->last_error = std::string("set_params: ") + why;
+            return BH_HIP_INVALID_ARGUMENT;
+        }
+    }
+    h->p = next;
     h->p.waves_per_cu = old.waves_per_cu;  // residency is fixed at creation
     h->leaf_depth = reference_leaf_depth(h->d.n, h->p.leaf_capacity);
     h->primed = false;
@@ -652,11 +692,19 @@ bh_hip_result_t bh_hip_dump_tree(bh_hip_handle_t* h, uint64_t* sorted_keys, uint
               cp(node_start, d.nstart, nn * 4, "start") && cp(node_count, d.ncount, nn * 4, "count") &&
               cp(node_nchild, d.nchild, nn * 4, "nchild") &&
               cp(node_moments, d.wnode, nn * sizeof(bh::WalkNode), "moments");
-    if (ok && node_level) {
+    if (ok && (node_level || node_moments)) {
         std::vector<uint8_t> lv(nn);
-        ok = cp(lv.data(), d.nlevel, nn, "level");
-        for (uint64_t i = 0; ok && i < nn; ++i)
+        bh::RootGeom g{};
+        ok = cp(lv.data(), d.nlevel, nn, "level") && cp(&g, d.geom, sizeof(g), "geometry");
+        for (uint64_t i = 0; ok && node_level && i < nn; ++i)
             node_level[i] = lv[i];
+        // record slots 10, 11: the cell size (from the level) and trs
+        const float wmax = std::max(std::max(g.width_f[0], g.width_f[1]), g.width_f[2]);
+        for (uint64_t i = 0; ok && node_moments && i < nn; ++i) {
+            float* rec = node_moments + 16 * i;
+            rec[11] = rec[10];
+            rec[10] = bh::cell_size(wmax, lv[i]);
+        }
     }
     if (ok && node_first_child) {
         std::vector<uint32_t> f(nn), c(nn);
@@ -811,6 +859,334 @@ bh_hip_result_t bh_hip_sort_pairs(const uint64_t* keys, const uint32_t* vals, ui
         set_err(err, err_cap, tmp.last_error);
         return BH_HIP_INTERNAL_ERROR;
     }
+    return BH_HIP_OK;
+}
+
+}  // extern "C"
+
+// ---- learned far-field correction ------------------------------------------
+namespace {
+
+bool lff_alloc_core(bh_hip_handle* h)
+{
+    bh::LffState& L = h->lff;
+    if (L.params)
+        return true;
+    L.partial_blocks = 256;
+    if (!dalloc(h, &L.params, bh::kLffParams, "lff params") ||
+        !dalloc(h, &L.adam_m, bh::kLffParams, "lff adam m") ||
+        !dalloc(h, &L.adam_v, bh::kLffParams, "lff adam v") ||
+        !dalloc(h, &L.grad, bh::kLffParams, "lff grad") ||
+        !dalloc(h, &h->lff_train_params, bh::kLffParams, "lff train params") ||
+        !dalloc(h, &L.partial, static_cast<size_t>(L.partial_blocks) * bh::kLffParams,
+                "lff partial") ||
+        !dalloc(h, &L.loss_partial, L.partial_blocks, "lff loss partial") ||
+        !dalloc(h, &L.loss_acc, 4, "lff loss") || !dalloc(h, &L.bias, 144, "lff bias") ||
+        !dalloc(h, &L.tuple_counter, 4, "lff counter"))
+        return false;
+    short* frag = nullptr;
+    if (!dalloc(h, &frag, 24 * 64 * 4, "lff fragments"))
+        return false;
+    L.frag = frag;
+    // He-normal hidden weights, small output layer, zero biases (seeded).
+    std::vector<float> p(bh::kLffParams, 0.0f);
+    std::mt19937_64 rng(20261202ull);
+    std::normal_distribution<float> nd(0.0f, 1.0f);
+    const int NF = bh::kLffInputs, NH = bh::kLffHidden, NO = bh::kLffOutputs;
+    for (int i = 0; i < NF * NH; ++i)
+        p[i] = nd(rng) * std::sqrt(2.0f / 15.0f);
+    for (int i = 0; i < NH * NH; ++i)
+        p[NF * NH + NH + i] = nd(rng) * std::sqrt(2.0f / NH);
+    for (int i = 0; i < NH * NO; ++i)
+        p[NF * NH + NH + NH * NH + NH + i] = nd(rng) * 0.01f;
+    for (int k = 0; k < NH; ++k)  // the pad input carries no weight
+        p[(NF - 1) * NH + k] = 0.0f;
+    if (!check(h, hipMemcpy(L.params, p.data(), p.size() * 4, hipMemcpyHostToDevice), "lff init"))
+        return false;
+    bh::launch_lff_pack(h->d, L);
+    return check(h, hipStreamSynchronize(h->d.stream), "lff pack");
+}
+
+bool lff_alloc_lists(bh_hip_handle* h)
+{
+    bh::LffState& L = h->lff;
+    const int cap = h->lff_cfg.list_cap > 0 ? h->lff_cfg.list_cap : 192;
+    const size_t groups = h->d.n / 8 + 1024;
+    if (L.lists && L.list_cap == cap && L.list_groups == groups)
+        return true;
+    dfree(h, L.lists);
+    dfree(h, L.list_n);
+    L.lists = nullptr;
+    L.list_n = nullptr;
+    L.list_cap = cap;
+    L.list_groups = groups;
+    return dalloc(h, &L.lists, groups * static_cast<size_t>(cap), "lff lists") &&
+           dalloc(h, &L.list_n, groups, "lff list counts");
+}
+
+}  // namespace
+
+extern "C" {
+
+bh_hip_result_t bh_hip_lff_configure(bh_hip_handle_t* h, const bh_hip_lff_config_t* cfg)
+{
+    if (!h || !cfg || !(cfg->theta > 0.0f) || !(cfg->band > 0.0f) || cfg->list_cap < 0 ||
+        cfg->min_count < 0)
+        return BH_HIP_INVALID_ARGUMENT;
+    if (cfg->enabled) {
+        if (const char* why = lff_incompatible(h->p)) {
+            h->last_error = std::string("lff_configure: ") + why;
+            return BH_HIP_INVALID_ARGUMENT;
+        }
+    }
+    h->lff_cfg = *cfg;
+    if (h->lff_cfg.list_cap == 0)
+        h->lff_cfg.list_cap = 192;
+    if (!lff_alloc_core(h) || !lff_alloc_lists(h))
+        return BH_HIP_INTERNAL_ERROR;
+    h->primed = false;
+    return BH_HIP_OK;
+}
+
+bh_hip_result_t bh_hip_lff_collect(bh_hip_handle_t* h, int32_t group_stride, uint64_t max_tuples,
+                                   uint64_t* total)
+{
+    if (!h || group_stride < 1 || max_tuples == 0 || max_tuples > 0xffffffffull)
+        return BH_HIP_INVALID_ARGUMENT;
+    if (const char* why = lff_incompatible(h->p)) {
+        h->last_error = std::string("lff_collect: ") + why;
+        return BH_HIP_INVALID_ARGUMENT;
+    }
+    if (!lff_alloc_core(h) || !lff_alloc_lists(h))
+        return BH_HIP_INTERNAL_ERROR;
+    bh::LffState& L = h->lff;
+    if (!L.X) {
+        if (!dalloc(h, &L.X, max_tuples * bh::kLffInputs, "lff X") ||
+            !dalloc(h, &L.Y, max_tuples * bh::kLffOutputs, "lff Y"))
+            return BH_HIP_INTERNAL_ERROR;
+        L.tuple_cap = max_tuples;
+        bh::launch_fill_u32(L.tuple_counter, 0u, 1, h->d.stream);
+    }
+    if (!zero_err(h))
+        return BH_HIP_INTERNAL_ERROR;
+    begin_call(h);
+    queue_build(h, nullptr);
+    bh::WalkParams wp = walk_params(h->p);
+    wp.theta2 = h->lff_cfg.theta * h->lff_cfg.theta;
+    bh::launch_walk_record(h->d, wp, h->stats, L, h->lff_cfg.band, h->lff_cfg.min_count);
+    bh::launch_lff_tuples(h->d, L, group_stride, h->lff_cfg.band, wp.eps2, h->lff_cfg.min_count,
+                          static_cast<unsigned int>(L.tuple_cap));
+    const bh_hip_result_t r = finish(h, "lff_collect");
+    end_call(h);
+    h->primed = false;  // acc holds the uncorrected coarse walk
+    if (r != BH_HIP_OK)
+        return r;
+    unsigned int cnt = 0;
+    if (!check(h, hipMemcpy(&cnt, L.tuple_counter, 4, hipMemcpyDeviceToHost), "lff count"))
+        return BH_HIP_INTERNAL_ERROR;
+    if (total)
+        *total = std::min<uint64_t>(cnt, L.tuple_cap);
+    return BH_HIP_OK;
+}
+
+bh_hip_result_t bh_hip_lff_train(bh_hip_handle_t* h, int32_t epochs, int32_t batch, float lr,
+                                 uint64_t seed, float* loss_per_epoch)
+{
+    if (!h || epochs < 1 || batch < 64 || !(lr > 0.0f))
+        return BH_HIP_INVALID_ARGUMENT;
+    bh::LffState& L = h->lff;
+    if (!L.X || !L.params) {
+        h->last_error = "lff_train: no training tuples (call bh_hip_lff_collect first)";
+        return BH_HIP_NOT_READY;
+    }
+    unsigned int cnt = 0;
+    if (!check(h, hipMemcpy(&cnt, L.tuple_counter, 4, hipMemcpyDeviceToHost), "lff count"))
+        return BH_HIP_INTERNAL_ERROR;
+    const size_t n = std::min<size_t>(cnt, L.tuple_cap);
+    if (n < static_cast<size_t>(batch)) {
+        h->last_error = "lff_train: fewer tuples than one batch";
+        return BH_HIP_NOT_READY;
+    }
+    // Output scale: RMS of the targets.
+    std::vector<float> y(3 * n);
+    if (!check(h, hipMemcpy(y.data(), L.Y, y.size() * 4, hipMemcpyDeviceToHost), "lff Y"))
+        return BH_HIP_INTERNAL_ERROR;
+    double ss = 0.0;
+    for (float v : y)
+        ss += static_cast<double>(v) * v;
+    const float sigma = static_cast<float>(std::sqrt(ss / static_cast<double>(y.size()))) + 1e-30f;
+    // Training copy in normalised output units.
+    std::vector<float> p(bh::kLffParams);
+    if (!check(h, hipMemcpy(p.data(), L.params, p.size() * 4, hipMemcpyDeviceToHost), "lff p"))
+        return BH_HIP_INTERNAL_ERROR;
+    const int NF = bh::kLffInputs, NH = bh::kLffHidden, NO = bh::kLffOutputs;
+    const size_t off_w3 = NF * NH + NH + NH * NH + NH;
+    for (size_t i = off_w3; i < p.size(); ++i)
+        p[i] /= sigma;
+    if (!check(h, hipMemcpy(h->lff_train_params, p.data(), p.size() * 4, hipMemcpyHostToDevice),
+               "lff p") ||
+        !check(h, hipMemset(L.adam_m, 0, bh::kLffParams * 4), "adam") ||
+        !check(h, hipMemset(L.adam_v, 0, bh::kLffParams * 4), "adam"))
+        return BH_HIP_INTERNAL_ERROR;
+    if (h->lff_order_cap < n) {
+        dfree(h, h->lff_order);
+        h->lff_order = nullptr;
+        if (!dalloc(h, &h->lff_order, n, "lff order"))
+            return BH_HIP_INTERNAL_ERROR;
+        h->lff_order_cap = n;
+    }
+    std::vector<uint32_t> order(n);
+    std::iota(order.begin(), order.end(), 0u);
+    std::mt19937_64 rng(seed);
+    const size_t steps_per_epoch = n / static_cast<size_t>(batch);
+    const int nblocks = std::min(L.partial_blocks, (batch + 63) / 64);
+    int step = 0;
+    const size_t total_steps = steps_per_epoch * static_cast<size_t>(epochs);
+    for (int e = 0; e < epochs; ++e) {
+        std::shuffle(order.begin(), order.end(), rng);
+        if (!check(h, hipMemcpyAsync(h->lff_order, order.data(), n * 4, hipMemcpyHostToDevice,
+                                     h->d.stream),
+                   "lff order") ||
+            !check(h, hipMemsetAsync(L.loss_acc, 0, 4, h->d.stream), "lff loss"))
+            return BH_HIP_INTERNAL_ERROR;
+        for (size_t b = 0; b < steps_per_epoch; ++b) {
+            ++step;
+            const double frac = static_cast<double>(step) / static_cast<double>(total_steps);
+            const float lr_t = static_cast<float>(lr * (0.1 + 0.9 * 0.5 * (1.0 + std::cos(M_PI * frac))));
+            bh::launch_lff_grad(h->d, L, L.X, L.Y, h->lff_order, b * batch,
+                                static_cast<uint32_t>(batch), 1.0f / sigma, h->lff_train_params,
+                                nblocks);
+            bh::launch_lff_adam(h->d, L, nblocks, h->lff_train_params, nullptr, lr_t, step);
+        }
+        float lsum = 0.0f;
+        if (!check(h, hipGetLastError(), "lff train") ||
+            !check(h, hipStreamSynchronize(h->d.stream), "lff train") ||
+            !check(h, hipMemcpy(&lsum, L.loss_acc, 4, hipMemcpyDeviceToHost), "lff loss"))
+            return BH_HIP_INTERNAL_ERROR;
+        if (loss_per_epoch)
+            loss_per_epoch[e] = lsum / static_cast<float>(steps_per_epoch * batch);
+    }
+    // Deploy: fold the output scale back into W3/b3 and repack the fragments.
+    if (!check(h, hipMemcpy(p.data(), h->lff_train_params, p.size() * 4, hipMemcpyDeviceToHost),
+               "lff p"))
+        return BH_HIP_INTERNAL_ERROR;
+    for (size_t i = off_w3; i < p.size(); ++i)
+        p[i] *= sigma;
+    (void)NO;
+    return bh_hip_lff_set_params(h, p.data());
+}
+
+bh_hip_result_t bh_hip_lff_get_params(bh_hip_handle_t* h, float* params)
+{
+    if (!h || !params)
+        return BH_HIP_INVALID_ARGUMENT;
+    if (!lff_alloc_core(h))
+        return BH_HIP_INTERNAL_ERROR;
+    return check(h, hipMemcpy(params, h->lff.params, bh::kLffParams * 4, hipMemcpyDeviceToHost),
+                 "lff get params")
+               ? BH_HIP_OK
+               : BH_HIP_INTERNAL_ERROR;
+}
+
+bh_hip_result_t bh_hip_lff_set_params(bh_hip_handle_t* h, const float* params)
+{
+    if (!h || !params)
+        return BH_HIP_INVALID_ARGUMENT;
+    if (!lff_alloc_core(h))
+        return BH_HIP_INTERNAL_ERROR;
+    if (!check(h, hipMemcpy(h->lff.params, params, bh::kLffParams * 4, hipMemcpyHostToDevice),
+               "lff set params"))
+        return BH_HIP_INTERNAL_ERROR;
+    bh::launch_lff_pack(h->d, h->lff);
+    h->primed = false;
+    return check(h, hipStreamSynchronize(h->d.stream), "lff pack") ? BH_HIP_OK
+                                                                    : BH_HIP_INTERNAL_ERROR;
+}
+
+bh_hip_result_t bh_hip_lff_gradient(bh_hip_handle_t* h, const float* X, const float* Y, uint32_t n,
+                                    const float* params, float* grad, float* loss)
+{
+    if (!h || !X || !Y || !params || !grad || n == 0)
+        return BH_HIP_INVALID_ARGUMENT;
+    if (!lff_alloc_core(h))
+        return BH_HIP_INTERNAL_ERROR;
+    bh::LffState& L = h->lff;
+    float *dX = nullptr, *dY = nullptr, *dP = nullptr;
+    uint32_t* dO = nullptr;
+    std::vector<uint32_t> ord(n);
+    std::iota(ord.begin(), ord.end(), 0u);
+    const int nblocks = std::min<int>(L.partial_blocks, static_cast<int>((n + 63) / 64));
+    bool ok = dalloc(h, &dX, static_cast<size_t>(n) * bh::kLffInputs, "grad X") &&
+              dalloc(h, &dY, static_cast<size_t>(n) * bh::kLffOutputs, "grad Y") &&
+              dalloc(h, &dP, bh::kLffParams, "grad P") && dalloc(h, &dO, n, "grad order");
+    ok = ok && check(h, hipMemcpy(dX, X, static_cast<size_t>(n) * bh::kLffInputs * 4,
+                                  hipMemcpyHostToDevice), "grad X") &&
+         check(h, hipMemcpy(dY, Y, static_cast<size_t>(n) * bh::kLffOutputs * 4,
+                            hipMemcpyHostToDevice), "grad Y") &&
+         check(h, hipMemcpy(dP, params, bh::kLffParams * 4, hipMemcpyHostToDevice), "grad P") &&
+         check(h, hipMemcpy(dO, ord.data(), n * 4, hipMemcpyHostToDevice), "grad order") &&
+         check(h, hipMemset(L.loss_acc, 0, 4), "grad loss");
+    if (ok) {
+        bh::launch_lff_grad(h->d, L, dX, dY, dO, 0, n, 1.0f, dP, nblocks);
+        bh::launch_lff_adam(h->d, L, nblocks, dP, L.grad, 0.0f, 1);
+        float lsum = 0.0f;
+        ok = check(h, hipGetLastError(), "grad") && check(h, hipStreamSynchronize(h->d.stream), "grad") &&
+             check(h, hipMemcpy(grad, L.grad, bh::kLffParams * 4, hipMemcpyDeviceToHost), "grad") &&
+             check(h, hipMemcpy(&lsum, L.loss_acc, 4, hipMemcpyDeviceToHost), "grad loss");
+        if (loss)
+            *loss = lsum / static_cast<float>(n);
+    }
+    dfree(h, dX);
+    dfree(h, dY);
+    dfree(h, dP);
+    dfree(h, dO);
+    return ok ? BH_HIP_OK : BH_HIP_INTERNAL_ERROR;
+}
+
+bh_hip_result_t bh_hip_lff_forward(bh_hip_handle_t* h, const float* X, uint32_t n, float* Y)
+{
+    if (!h || !X || !Y || n == 0)
+        return BH_HIP_INVALID_ARGUMENT;
+    if (!lff_alloc_core(h))
+        return BH_HIP_INTERNAL_ERROR;
+    float *dX = nullptr, *dY = nullptr;
+    bool ok = dalloc(h, &dX, static_cast<size_t>(n) * bh::kLffInputs, "fwd X") &&
+              dalloc(h, &dY, static_cast<size_t>(n) * bh::kLffOutputs, "fwd Y");
+    ok = ok && check(h, hipMemcpy(dX, X, static_cast<size_t>(n) * bh::kLffInputs * 4,
+                                  hipMemcpyHostToDevice), "fwd X");
+    if (ok) {
+        bh::launch_lff_forward(h->d, h->lff, dX, n, dY);
+        ok = check(h, hipGetLastError(), "fwd") && check(h, hipStreamSynchronize(h->d.stream), "fwd") &&
+             check(h, hipMemcpy(Y, dY, static_cast<size_t>(n) * bh::kLffOutputs * 4,
+                                hipMemcpyDeviceToHost), "fwd Y");
+    }
+    dfree(h, dX);
+    dfree(h, dY);
+    return ok ? BH_HIP_OK : BH_HIP_INTERNAL_ERROR;
+}
+
+bh_hip_result_t bh_hip_lff_get_tuples(bh_hip_handle_t* h, float* X, float* Y, uint64_t cap,
+                                      uint64_t* n)
+{
+    if (!h)
+        return BH_HIP_INVALID_ARGUMENT;
+    unsigned int cnt = 0;
+    if (h->lff.X && !check(h, hipMemcpy(&cnt, h->lff.tuple_counter, 4, hipMemcpyDeviceToHost),
+                           "lff count"))
+        return BH_HIP_INTERNAL_ERROR;
+    const uint64_t avail = std::min<uint64_t>(cnt, h->lff.tuple_cap);
+    if (!X && !Y) {  // count query
+        if (n)
+            *n = avail;
+        return BH_HIP_OK;
+    }
+    const uint64_t m = std::min<uint64_t>(avail, cap);
+    if (m && X && !check(h, hipMemcpy(X, h->lff.X, m * bh::kLffInputs * 4, hipMemcpyDeviceToHost), "X"))
+        return BH_HIP_INTERNAL_ERROR;
+    if (m && Y && !check(h, hipMemcpy(Y, h->lff.Y, m * bh::kLffOutputs * 4, hipMemcpyDeviceToHost), "Y"))
+        return BH_HIP_INTERNAL_ERROR;
+    if (n)
+        *n = m;
     return BH_HIP_OK;
 }
 

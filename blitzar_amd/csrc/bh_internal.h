@@ -16,6 +16,13 @@ constexpr int kInfoTotal = 2 * kLevels;      // level_info slot: total nodes
 constexpr int kInfoMaxLevel = 2 * kLevels + 1;  // deepest populated level
 constexpr int kInfoGroups = 2 * kLevels + 2;    // number of walk groups
 
+// Learned far-field MLP: 16 inputs (15 features + pad) -> 64 -> 64 -> 3.
+constexpr int kLffInputs = 16;
+constexpr int kLffHidden = 64;
+constexpr int kLffOutputs = 3;
+constexpr int kLffParams = kLffInputs * kLffHidden + kLffHidden + kLffHidden * kLffHidden +
+                           kLffHidden + kLffHidden * kLffOutputs + kLffOutputs;  // 5443
+
 // Root geometry, written on the device once per build by geometry_kernel.
 //   FP32 part: the quantisation frame of the key kernel (AABB of the
 //   float-cast positions and its per-axis reciprocal extent).
@@ -39,19 +46,40 @@ struct RootGeom {
 //   q                 : traceless quadrupole about the COM (xx, xy, xz, yy, yz;
 //                       zz = -xx - yy)
 //   bmax              : radius about the COM enclosing every particle
-//   size              : largest cell width at this level (opening criterion)
+//   trs               : trace of the second moment about the COM,
+//                       sum m |x - com|^2; with Plummer softening eps the
+//                       second-order expansion carries -5/2 eps^2 trs d / r^7
 //   first             : first child node (internal) or first sorted particle (leaf)
+// The cell size used by the opening criterion follows from the level
+// (cell_size below).
 //   cx, cy, cz        : lattice centre
-//   meta              : nchild (bits 0-3) | level (bits 4-8) | count (bits 9-31)
+//   meta              : nchild (bits 0-3) | level (bits 4-8) | count (bits 9-31,
+//                       particles in the subtree, saturated at 2^23 - 1)
 struct alignas(16) WalkNode {
     float comx, comy, comz, mass;
     float qxx, qxy, qxz, qyy;
-    float qyz, bmax, size;
+    float qyz, bmax, trs;
     int first;
     int cx, cy, cz;
     uint32_t meta;
 };
 static_assert(sizeof(WalkNode) == 64, "WalkNode must be 64 bytes");
+
+// Position relative to a reference lattice centre: the lattice part
+// dc * unit is rounded on its own before the offset is added (no fused
+// multiply-add), so every kernel forms the same bits for the same particle and
+// a target meets itself as a source at exactly zero separation.
+__host__ __device__ inline float rel_pos(int dc, float unit, float off)
+{
+#pragma clang fp contract(off)
+    return static_cast<float>(dc) * unit + off;
+}
+
+// Largest cell width at `level`: wmax = max finest cell width over the axes.
+__host__ __device__ inline float cell_size(float wmax, int level)
+{
+    return wmax * ldexpf(1.0f, kMaxLevel - level);
+}
 
 __host__ __device__ inline uint32_t meta_nchild(uint32_t m) { return m & 15u; }
 __host__ __device__ inline uint32_t meta_level(uint32_t m) { return (m >> 4) & 31u; }
@@ -141,6 +169,28 @@ struct Device {
     size_t diag_cap = 0;
 };
 
+// Learned far-field state of one solver instance (device pointers).
+struct LffState {
+    uint32_t* lists = nullptr;    // per group: recorded node ids, list_cap each
+    uint32_t* list_n = nullptr;   // per group: number of recorded nodes
+    int list_cap = 0;
+    size_t list_groups = 0;       // groups the lists can hold
+    float* params = nullptr;      // kLffParams FP32
+    float* adam_m = nullptr;
+    float* adam_v = nullptr;
+    float* grad = nullptr;        // kLffParams
+    float* partial = nullptr;     // partial_blocks x kLffParams
+    float* loss_partial = nullptr;
+    float* loss_acc = nullptr;
+    int partial_blocks = 0;
+    void* frag = nullptr;         // 24 x 64 BF16x4 MFMA fragments
+    float* bias = nullptr;        // b1[64], b2[64], b3[16]
+    float* X = nullptr;           // training features, tuple_cap x 16
+    float* Y = nullptr;           // training targets, tuple_cap x 3
+    unsigned int* tuple_counter = nullptr;
+    size_t tuple_cap = 0;
+};
+
 // ---- launchers (defined in the kernel translation units) -------------------
 // k_keys.hip
 void launch_aabb(Device& d);
@@ -169,6 +219,25 @@ void launch_groups(Device& d);
 // k_walk.hip
 // stats (optional, device): [0] particle-particle, [1] particle-cell interactions
 void launch_walk(Device& d, const WalkParams& p, unsigned long long* stats);
+// Walk of the learned correction. A node that passes the opening test and is
+// near the group, size^2 >= band^2 * (distance from its COM to the group
+// box)^2, is opened when it holds fewer than min_count particles (so small
+// near nodes are summed at the finer angle `band`), else accepted and
+// recorded in L.lists for the MLP pass.
+void launch_walk_record(Device& d, const WalkParams& p, unsigned long long* stats,
+                        const LffState& L, float band, int min_count);
+// k_lff.hip
+void launch_lff_tuples(Device& d, const LffState& L, int group_stride, float band, float eps2,
+                       int min_count, unsigned int max_tuples);
+void launch_lff_grad(Device& d, const LffState& L, const float* X, const float* Y,
+                     const uint32_t* order, size_t first, uint32_t nbatch, float yscale,
+                     const float* P, int nblocks);
+void launch_lff_adam(Device& d, const LffState& L, int nblocks, float* P, float* grad, float lr,
+                     int step);
+void launch_lff_pack(Device& d, const LffState& L);
+void launch_lff_forward(Device& d, const LffState& L, const float* X, uint32_t n, float* Y);
+void launch_lff_apply(Device& d, const LffState& L, float band, float eps2, int min_count,
+                      float max_acc);
 // k_integrate.hip
 void launch_kick(Device& d, double dt_kick, double max_speed);
 void launch_kick_drift(Device& d, double dt_kick, double dt_drift, double max_speed);

@@ -208,10 +208,7 @@ __global__ __launch_bounds__(256) void tree_moments_kernel(
 {
     const int base = info[level];
     const int cnt = info[kLevels + level];
-    const float wx = geom->width_f[0], wy = geom->width_f[1], wz = geom->width_f[2];
     const float ux = geom->unit_f[0], uy = geom->unit_f[1], uz = geom->unit_f[2];
-    const float lscale = ldexpf(1.0f, kMaxLevel - level);
-    const float size = fmaxf(fmaxf(wx, wy), wz) * lscale;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
         const int node = base + k;
         int c[3];
@@ -219,7 +216,7 @@ __global__ __launch_bounds__(256) void tree_moments_kernel(
         const uint32_t nc = nchild[node];
         const uint32_t first = nfirst[node];
         float M = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
-        float qxx = 0.0f, qxy = 0.0f, qxz = 0.0f, qyy = 0.0f, qyz = 0.0f, bmax = 0.0f;
+        float qxx = 0.0f, qxy = 0.0f, qxz = 0.0f, qyy = 0.0f, qyz = 0.0f, bmax = 0.0f, trs = 0.0f;
         uint32_t count = 0;
         if (nc == 0) {
             count = ncount[node];
@@ -253,6 +250,7 @@ __global__ __launch_bounds__(256) void tree_moments_kernel(
                 qxz += p.w * (3.0f * dx * dz);
                 qyy += p.w * (3.0f * dy * dy - r2);
                 qyz += p.w * (3.0f * dy * dz);
+                trs += p.w * r2;
                 bmax = fmaxf(bmax, r2);
             }
             bmax = sqrtf(bmax);
@@ -284,6 +282,7 @@ __global__ __launch_bounds__(256) void tree_moments_kernel(
                 qxz += ch.qxz + ch.mass * (3.0f * dx * dz);
                 qyy += ch.qyy + ch.mass * (3.0f * dy * dy - r2);
                 qyz += ch.qyz + ch.mass * (3.0f * dy * dz);
+                trs += ch.trs + ch.mass * r2;
                 if (ch.mass > 0.0f)
                     bmax = fmaxf(bmax, sqrtf(r2) + ch.bmax);
             }
@@ -299,12 +298,16 @@ __global__ __launch_bounds__(256) void tree_moments_kernel(
         w.qyy = qyy;
         w.qyz = qyz;
         w.bmax = bmax;
-        w.size = size;
+        w.trs = trs;
         w.first = static_cast<int>(first);
         w.cx = c[0];
         w.cy = c[1];
         w.cz = c[2];
-        if (count > kMaxLeafCount) {
+        if (nc != 0) {
+            // internal node: the subtree's particle count, saturated (the walk
+            // reads it only to pick the near-field treatment)
+            count = min(ncount[node], kMaxLeafCount);
+        } else if (count > kMaxLeafCount) {
             // The walk then sees a truncated leaf: flag it; every particle was
             // still packed above, so no access goes out of bounds.
             atomicOr(err, kErrLeafTooLarge);

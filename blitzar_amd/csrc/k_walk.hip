@@ -27,6 +27,13 @@
 // centre of the group's node: float(c_node - c_ref) * unit + offset,
 // with integer lattice coordinates c, so the walk itself uses no FP64.
 //
+// Second order (multipole_order 2) is the expansion of the Plummer-softened
+// kernel g(r) = (r^2 + eps^2)^-1/2 about the COM, d = com - x, R^2 = |d|^2 + eps^2:
+//   a = M d / R^3 - Q d / R^5 + 5/2 (d.Q d - eps^2 trs) d / R^7,
+// Q the traceless quadrupole, trs = sum m |y|^2 (the eps^2 trs term is what
+// the traceless Q cannot carry once eps > 0). multipole_order 1 is the
+// reference's softened monopole.
+//
 // Cost model for the roofline: 20 FLOP per particle-particle and 50 FLOP per
 // particle-cell (monopole + quadrupole) interaction.
 #include "bh_internal.h"
@@ -89,13 +96,13 @@ __device__ inline void eval_cells(const float4* __restrict__ ca, const float4* _
         if (ORDER >= 2) {
             const float4 b = cb[j];
             const float2 c = cc[j];
-            // b = (qxx, qxy, qxz, qyy), c = (qyz, qzz)
+            // b = (qxx, qxy, qxz, qyy), c = (qyz, eps^2 trs); qzz = -qxx - qyy
             const float qx = b.x * dx + b.y * dy + b.z * dz;
             const float qy = b.y * dx + b.w * dy + c.x * dz;
-            const float qz = b.z * dx + c.x * dy + c.y * dz;
+            const float qz = b.z * dx + c.x * dy - (b.x + b.w) * dz;
             const float dqd = dx * qx + dy * qy + dz * qz;
             const float ri5 = ri3 * ri2;
-            const float s = (a.w * ri3 + 2.5f * dqd * ri5 * ri2) * wgt;
+            const float s = (a.w * ri3 + 2.5f * (dqd - c.y) * ri5 * ri2) * wgt;
             const float t = ri5 * wgt;
             ax += s * dx - t * qx;
             ay += s * dy - t * qy;
@@ -134,14 +141,16 @@ __device__ inline void eval_parts(const float4* __restrict__ pl, int np, float t
     }
 }
 
-template <int ORDER, int CRIT, bool TPM>
+template <int ORDER, int CRIT, bool TPM, bool REC>
 __global__ __launch_bounds__(64) void walk_kernel(
     WalkParams P, const WalkNode* __restrict__ nodes, const float4* __restrict__ spart,
     const uint32_t* __restrict__ sleaf, const RootGeom* __restrict__ geom,
     const uint4* __restrict__ groups, const int* __restrict__ info,
     int node_cap, unsigned int* __restrict__ work, int* __restrict__ stack_all, int stack_cap,
     float* __restrict__ accx, float* __restrict__ accy, float* __restrict__ accz,
-    int* __restrict__ err, unsigned long long* __restrict__ stats)
+    int* __restrict__ err, unsigned long long* __restrict__ stats, uint32_t* __restrict__ rec_lists,
+    uint32_t* __restrict__ rec_n, int rec_cap, int rec_groups, float rec_band2,
+    uint32_t rec_min_count)
 {
     __shared__ float4 cA[kCellCap];
     __shared__ float4 cB[kCellCap];
@@ -156,6 +165,7 @@ __global__ __launch_bounds__(64) void walk_kernel(
     int* stk = stack_all + static_cast<size_t>(blockIdx.x) * static_cast<size_t>(stack_cap);
     const float ux = geom->unit_f[0], uy = geom->unit_f[1], uz = geom->unit_f[2];
     const float wfx = geom->width_f[0], wfy = geom->width_f[1], wfz = geom->width_f[2];
+    const float wfmax = fmaxf(fmaxf(wfx, wfy), wfz);
     const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     unsigned long long npp = 0, npc = 0;
     const int ngroups = info[kInfoGroups];
@@ -180,9 +190,9 @@ __global__ __launch_bounds__(64) void walk_kernel(
         if (active) {
             const float4 p = spart[s];
             const int lf = static_cast<int>(sleaf[s]);
-            tx = p.x + static_cast<float>(nodes[lf].cx - rcx) * ux;
-            ty = p.y + static_cast<float>(nodes[lf].cy - rcy) * uy;
-            tz = p.z + static_cast<float>(nodes[lf].cz - rcz) * uz;
+            tx = rel_pos(nodes[lf].cx - rcx, ux, p.x);
+            ty = rel_pos(nodes[lf].cy - rcy, uy, p.y);
+            tz = rel_pos(nodes[lf].cz - rcz, uz, p.z);
         }
         const float gx0 = wmin(active ? tx : 3.0e38f), gx1 = wmax(active ? tx : -3.0e38f);
         const float gy0 = wmin(active ? ty : 3.0e38f), gy1 = wmax(active ? ty : -3.0e38f);
@@ -192,6 +202,7 @@ __global__ __launch_bounds__(64) void walk_kernel(
 
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
         int ncell = 0, npart = 0;
+        int nrec = 0;
         int sp = 1;
         if (lane == 0)
             stk[0] = 0;  // root
@@ -220,6 +231,8 @@ __global__ __launch_bounds__(64) void walk_kernel(
             WalkNode w = {};
             float sx = 0.0f, sy = 0.0f, sz = 0.0f;
             float px = 0.0f, py = 0.0f, pz = 0.0f;
+            float size = 0.0f;
+            bool near = false;
             if (node >= 0) {
                 w = nodes[node];
                 if (w.mass > 0.0f) {
@@ -234,16 +247,17 @@ __global__ __launch_bounds__(64) void walk_kernel(
                     const float dz = fmaxf(fabsf(pz - gcz) - ghz, 0.0f);
                     const float d2 = dx * dx + dy * dy + dz * dz;
                     bool ok = d2 > w.bmax * w.bmax;
+                    const int lvl = static_cast<int>(meta_level(w.meta));
+                    size = cell_size(wfmax, lvl);
                     if (CRIT == 1) {
-                        const int lvl = static_cast<int>(meta_level(w.meta));
                         const float hs = ldexpf(0.5f, kMaxLevel - lvl);
                         const float bx = fmaxf(fabsf(sx - gcx) - ghx - hs * wfx, 0.0f);
                         const float by = fmaxf(fabsf(sy - gcy) - ghy - hs * wfy, 0.0f);
                         const float bz = fmaxf(fabsf(sz - gcz) - ghz - hs * wfz, 0.0f);
                         const float b2 = bx * bx + by * by + bz * bz;
-                        ok = ok && b2 > 0.0f && w.size * w.size < P.theta2 * b2;
+                        ok = ok && b2 > 0.0f && size * size < P.theta2 * b2;
                     } else {
-                        ok = ok && w.size * w.size < P.theta2 * (d2 + P.eps2);
+                        ok = ok && size * size < P.theta2 * (d2 + P.eps2);
                     }
                     if (TPM && P.rcut2 > 0.0f) {
                         // Prune nodes whose bounding sphere lies beyond the cutoff for the
@@ -252,6 +266,15 @@ __global__ __launch_bounds__(64) void walk_kernel(
                         if (d2 > lim * lim) {
                             ok = false;
                             w.mass = 0.0f;
+                        }
+                    }
+                    if (REC && ok) {
+                        // near the group: small nodes are opened, large ones
+                        // accepted and recorded for the learned correction
+                        near = size * size >= rec_band2 * d2;
+                        if (near && meta_count(w.meta) < rec_min_count) {
+                            ok = false;
+                            near = false;
                         }
                     }
                     if (w.mass > 0.0f) {
@@ -263,13 +286,25 @@ __global__ __launch_bounds__(64) void walk_kernel(
                 }
             }
 
+            if (REC) {
+                near = near && accept;
+                const unsigned long long rm = __ballot(near);
+                if (near && g < rec_groups) {
+                    const int slot = nrec + static_cast<int>(__popcll(rm & lt));
+                    if (slot < rec_cap)
+                        rec_lists[static_cast<size_t>(g) * rec_cap + slot] =
+                            static_cast<uint32_t>(node);
+                }
+                nrec += static_cast<int>(__popcll(rm));
+            }
+
             // 1) accepted nodes -> LDS cell list
             const unsigned long long am = __ballot(accept);
             if (accept) {
                 const int slot = ncell + static_cast<int>(__popcll(am & lt));
                 cA[slot] = make_float4(px, py, pz, w.mass);
                 cB[slot] = make_float4(w.qxx, w.qxy, w.qxz, w.qyy);
-                cC[slot] = make_float2(w.qyz, -w.qxx - w.qyy);
+                cC[slot] = make_float2(w.qyz, P.eps2 * w.trs);
             }
             ncell += static_cast<int>(__popcll(am));
 
@@ -354,6 +389,8 @@ __global__ __launch_bounds__(64) void walk_kernel(
         }
         npp = 0;
         npc = 0;
+        if (REC && lane == 0 && g < rec_groups)
+            rec_n[g] = static_cast<uint32_t>(nrec < rec_cap ? nrec : rec_cap);
         __syncthreads();
 
         if (active) {
@@ -372,16 +409,33 @@ __global__ __launch_bounds__(64) void walk_kernel(
     }
 }
 
-template <int ORDER, int CRIT, bool TPM>
-void launch_walk_t(Device& d, const WalkParams& p, unsigned grid, unsigned long long* stats)
+template <int ORDER, int CRIT, bool TPM, bool REC = false>
+void launch_walk_t(Device& d, const WalkParams& p, unsigned grid, unsigned long long* stats,
+                   const LffState* L = nullptr, float band = 0.0f, int min_count = 0)
 {
-    walk_kernel<ORDER, CRIT, TPM><<<grid, 64, 0, d.stream>>>(
+    walk_kernel<ORDER, CRIT, TPM, REC><<<grid, 64, 0, d.stream>>>(
         p, d.wnode, d.spart, d.sleaf, d.geom, d.groups, d.level_info,
         static_cast<int>(d.node_cap), d.work_counter,
-        d.stack, static_cast<int>(d.stack_cap), d.acc[0], d.acc[1], d.acc[2], d.err, stats);
+        d.stack, static_cast<int>(d.stack_cap), d.acc[0], d.acc[1], d.acc[2], d.err, stats,
+        L ? L->lists : nullptr, L ? L->list_n : nullptr, L ? L->list_cap : 0,
+        L ? static_cast<int>(L->list_groups) : 0, band * band,
+        static_cast<uint32_t>(min_count > 0 ? min_count : 0));
 }
 
 }  // namespace
+
+void launch_walk_record(Device& d, const WalkParams& p, unsigned long long* stats,
+                        const LffState& L, float band, int min_count)
+{
+    if (d.n == 0)
+        return;
+    launch_fill_u32(d.work_counter, 0u, 1, d.stream);
+    const unsigned grid = static_cast<unsigned>(d.stack_waves);
+    if (p.criterion == 1)
+        launch_walk_t<2, 1, false, true>(d, p, grid, stats, &L, band, min_count);
+    else
+        launch_walk_t<2, 0, false, true>(d, p, grid, stats, &L, band, min_count);
+}
 
 void launch_walk(Device& d, const WalkParams& p, unsigned long long* stats)
 {

@@ -194,6 +194,79 @@ class OctreeHip:
         return {k: getattr(t, k) for k, _ in Timings._fields_}
 
 
+    # -- learned far-field correction -----------------------------------------
+    def lff_configure(self, theta: float = 0.9, band: float = 0.5, list_cap: int = 192,
+                      enabled: bool = False, min_count: int = 32) -> None:
+        """Set the corrected walk's opening angle, near band and the smallest
+        corrected node; enable it."""
+        cfg = _abi.LffConfig(theta, band, list_cap, 1 if enabled else 0, min_count)
+        _check(lib().bh_hip_lff_configure(self._h, C.byref(cfg)), self._h, "lff_configure")
+        self._lff_cfg = cfg
+
+    def lff_enable(self, enabled: bool = True) -> None:
+        cfg = getattr(self, "_lff_cfg", None) or _abi.LffConfig(0.9, 0.5, 192, 0, 32)
+        cfg.enabled = 1 if enabled else 0
+        _check(lib().bh_hip_lff_configure(self._h, C.byref(cfg)), self._h, "lff_configure")
+        self._lff_cfg = cfg
+
+    def lff_collect(self, group_stride: int = 1, max_tuples: int = 1 << 22) -> int:
+        """Walk at the correction's theta and append training tuples; returns the count."""
+        total = C.c_uint64()
+        _check(lib().bh_hip_lff_collect(self._h, group_stride, max_tuples, C.byref(total)),
+               self._h, "lff_collect")
+        return int(total.value)
+
+    def lff_train(self, epochs: int = 20, batch: int = 4096, lr: float = 2e-3,
+                  seed: int = 1) -> np.ndarray:
+        """Train on the device tuples; returns the loss per epoch."""
+        loss = np.zeros(epochs, np.float32)
+        _check(lib().bh_hip_lff_train(self._h, epochs, batch, lr, seed, loss), self._h,
+               "lff_train")
+        return loss
+
+    def lff_params(self) -> np.ndarray:
+        p = np.zeros(_abi.LFF_PARAMS, np.float32)
+        _check(lib().bh_hip_lff_get_params(self._h, p), self._h, "lff_get_params")
+        return p
+
+    def lff_set_params(self, params) -> None:
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != _abi.LFF_PARAMS:
+            raise ValueError(f"expected {_abi.LFF_PARAMS} parameters, got {p.size}")
+        _check(lib().bh_hip_lff_set_params(self._h, p), self._h, "lff_set_params")
+
+    def lff_gradient(self, X, Y, params):
+        """(gradient, loss) of mean |MLP(X) - Y|^2 through the FP32 training kernels."""
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.ascontiguousarray(Y, dtype=np.float32)
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        g = np.zeros(_abi.LFF_PARAMS, np.float32)
+        loss = C.c_float()
+        _check(lib().bh_hip_lff_gradient(self._h, X, Y, X.shape[0], p, g, C.byref(loss)),
+               self._h, "lff_gradient")
+        return g, float(loss.value)
+
+    def lff_forward(self, X) -> np.ndarray:
+        """MLP outputs for feature rows through the BF16 MFMA inference kernel."""
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((X.shape[0], _abi.LFF_OUTPUTS), np.float32)
+        _check(lib().bh_hip_lff_forward(self._h, X, X.shape[0], Y), self._h, "lff_forward")
+        return Y
+
+    def lff_tuples(self, cap: int | None = None):
+        """Copy the training tuples (features, normalised residuals) to the host."""
+        n = C.c_uint64()
+        _check(lib().bh_hip_lff_get_tuples(self._h, None, None, 0, C.byref(n)), self._h,
+               "lff_get_tuples")
+        m = int(n.value) if cap is None else min(int(n.value), cap)
+        X = np.zeros((m, _abi.LFF_INPUTS), np.float32)
+        Y = np.zeros((m, _abi.LFF_OUTPUTS), np.float32)
+        if m:
+            _check(lib().bh_hip_lff_get_tuples(self._h, X, Y, m, C.byref(n)), self._h,
+                   "lff_get_tuples")
+        return X, Y
+
+
 def sort_pairs(keys, vals, key_bits: int = 64):
     """Device stable radix sort of (uint64 key, uint32 value) pairs."""
     k = np.ascontiguousarray(keys, np.uint64)

@@ -136,8 +136,9 @@ bh_hip_result_t bh_hip_tree_info(bh_hip_handle_t* h, uint64_t* node_count, int32
  *   node_prefix/level/start/count/first_child/nchild : breadth-first nodes
  *                            (first_child = -1 for leaves)
  *   node_moments           : 16 floats per node: com offset xyz, mass,
- *                            qxx qxy qxz qyy qyz, bmax, size, (first),
- *                            lattice centre xyz, (meta)  (raw 64-byte records)
+ *                            qxx qxy qxz qyy qyz (traceless, about the COM),
+ *                            bmax, cell size, trs = sum m |x - com|^2,
+ *                            lattice centre xyz (int32), meta (uint32)
  *   geom                   : 12 doubles: FP32 frame lo[3], inv[3]; lattice
  *                            origin[3], unit[3] */
 bh_hip_result_t bh_hip_dump_tree(bh_hip_handle_t* h, uint64_t* sorted_keys, uint32_t* sorted_ids,
@@ -165,6 +166,51 @@ bh_hip_result_t bh_hip_get_timings(bh_hip_handle_t* h, bh_hip_timings_t* out);
 
 /* Last error message of the handle (FfiApi.hpp:83-100 style). */
 bh_hip_result_t bh_hip_get_last_error(bh_hip_handle_t* h, char* buf, size_t cap);
+
+/* ---- Learned far-field correction --------------------------------------
+ * No counterpart in the reference. A small MLP (16 -> 64 -> 64 -> 3, SiLU,
+ * BF16 MFMA inference with FP32 accumulation) predicts, for an accepted node
+ * of at least min_count particles close to a target (size / distance >=
+ * band), the residual of the second-order expansion in units of M / r^2; the
+ * corrected walk runs at a larger opening angle. Training tuples come from this solver's own walks
+ * (exact residual = direct sum over the node's particles) and the network is
+ * trained on the GPU by this library's forward/backward kernels (Adam). */
+typedef struct bh_hip_lff_config {
+    float theta;       /* opening angle of the corrected walk (e.g. 0.9) */
+    float band;        /* near nodes: size / distance >= band */
+    int32_t list_cap;  /* nodes recorded per target group (0 = default 192) */
+    int32_t enabled;   /* 1: force evaluations and steps use the corrected walk */
+    int32_t min_count; /* near nodes with fewer particles are opened (summed at
+                          the angle `band`) instead of corrected (default 32) */
+} bh_hip_lff_config_t;
+
+#define BH_HIP_LFF_INPUTS 16
+#define BH_HIP_LFF_PARAMS 5443 /* W1[16][64] b1[64] W2[64][64] b2[64] W3[64][3] b3[3] */
+
+bh_hip_result_t bh_hip_lff_configure(bh_hip_handle_t* h, const bh_hip_lff_config_t* cfg);
+/* Build the tree at the current positions, walk at cfg->theta recording the
+ * near nodes of every group_stride-th group, and append their (features,
+ * normalised residual) tuples to the device training set (holding at most
+ * max_tuples; allocated on the first call). *total receives the tuple count. */
+bh_hip_result_t bh_hip_lff_collect(bh_hip_handle_t* h, int32_t group_stride, uint64_t max_tuples,
+                                   uint64_t* total);
+/* Train on the device set: epochs x (n / batch) Adam steps, learning rate lr
+ * decayed by a cosine to lr / 10; loss_per_epoch (may be NULL) receives the
+ * mean squared error per tuple in units of the target RMS. */
+bh_hip_result_t bh_hip_lff_train(bh_hip_handle_t* h, int32_t epochs, int32_t batch, float lr,
+                                 uint64_t seed, float* loss_per_epoch);
+bh_hip_result_t bh_hip_lff_get_params(bh_hip_handle_t* h, float* params);
+bh_hip_result_t bh_hip_lff_set_params(bh_hip_handle_t* h, const float* params);
+/* Gradient of the mean over n samples of |MLP(x) - y|^2 with the given
+ * parameters (FP32 training kernels); X is n x 16, Y is n x 3. */
+bh_hip_result_t bh_hip_lff_gradient(bh_hip_handle_t* h, const float* X, const float* Y, uint32_t n,
+                                    const float* params, float* grad, float* loss);
+/* MLP outputs for n feature rows through the BF16 MFMA inference kernel. */
+bh_hip_result_t bh_hip_lff_forward(bh_hip_handle_t* h, const float* X, uint32_t n, float* Y);
+/* Copy up to cap training tuples to the host; *n receives the count copied.
+ * With X and Y both NULL, *n receives the number of tuples held. */
+bh_hip_result_t bh_hip_lff_get_tuples(bh_hip_handle_t* h, float* X, float* Y, uint64_t cap,
+                                      uint64_t* n);
 
 /* Standalone stable radix sort of (key, value) pairs on the device, the
  * building block behind thrust::sort_by_key at OctBuild.inl:96. Sorts by the

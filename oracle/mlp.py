@@ -1,0 +1,183 @@
+"""FP64 restatement of the learned far-field correction (TEST INFRASTRUCTURE ONLY).
+
+Only tests/ may import this module, as the checker of the GPU kernels in
+blitzar_amd/csrc/k_lff.hip. The learned correction has no counterpart in the
+reference (octree_cpu and octree_gpu evaluate plain multipoles), so nothing
+here follows a reference file; every function is pinned in
+tests/test_oracle_mlp.py by what the mathematics fixes:
+
+* `loss_grad`     central finite differences of `loss` over every parameter,
+                  and the closed form of the all-zero-weights network;
+* `to_bf16`       agrees with torch's float32 -> bfloat16 cast (round to
+                  nearest, ties to even) and with hand-worked ties;
+* `forward_bf16`  stays within BF16 rounding of `forward` (it is the tighter
+                  model of the inference kernel; `forward` bounds it);
+* `multipole_accel`
+                  with eps > 0 it equals the second-order Taylor expansion of
+                  the softened pair force of a two-particle node (closed form
+                  m [2 f(D) + a^2 f''(D)], f(z) = z (z^2 + eps^2)^-3/2);
+* `node_features` / `node_residual`
+                  the residual of a single-particle node is exactly zero, a
+                  two-particle node matches its closed-form octupole-leading
+                  residual, the residual of a random node falls off as
+                  (size / distance)^3 relative to the monopole, and features
+                  and residuals are invariant under translation, scaling of
+                  lengths (eps included) and scaling of masses.
+
+Network: x[16] -> SiLU(x W1 + b1)[64] -> SiLU(h1 W2 + b2)[64] -> h2 W3 + b3 [3].
+Packed parameters (5443 floats): W1[16][64] b1[64] W2[64][64] b2[64] W3[64][3] b3[3].
+The loss of a batch is mean over samples of sum over the 3 outputs of (out - y)^2.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+NF, NH, NO = 16, 64, 3
+OFF_W1 = 0
+OFF_B1 = OFF_W1 + NF * NH
+OFF_W2 = OFF_B1 + NH
+OFF_B2 = OFF_W2 + NH * NH
+OFF_W3 = OFF_B2 + NH
+OFF_B3 = OFF_W3 + NH * NO
+NP = OFF_B3 + NO
+assert NP == 5443
+
+
+def unpack(p):
+    p = np.asarray(p, np.float64)
+    return (p[OFF_W1:OFF_B1].reshape(NF, NH), p[OFF_B1:OFF_W2], p[OFF_W2:OFF_B2].reshape(NH, NH),
+            p[OFF_B2:OFF_W3], p[OFF_W3:OFF_B3].reshape(NH, NO), p[OFF_B3:NP])
+
+
+def pack(W1, b1, W2, b2, W3, b3) -> np.ndarray:
+    return np.concatenate([np.ravel(W1), b1, np.ravel(W2), b2, np.ravel(W3), b3])
+
+
+def _sigmoid(z):
+    return 1.0 / (1.0 + np.exp(-z))
+
+
+def silu(z):
+    return z * _sigmoid(z)
+
+
+def dsilu(z):
+    s = _sigmoid(z)
+    return s * (1.0 + z * (1.0 - s))
+
+
+def forward(p, X) -> np.ndarray:
+    W1, b1, W2, b2, W3, b3 = unpack(p)
+    h1 = silu(np.asarray(X, np.float64) @ W1 + b1)
+    h2 = silu(h1 @ W2 + b2)
+    return h2 @ W3 + b3
+
+
+def loss(p, X, Y) -> float:
+    e = forward(p, X) - np.asarray(Y, np.float64)
+    return float(np.mean(np.sum(e * e, axis=1)))
+
+
+def loss_grad(p, X, Y):
+    """(loss, gradient) by reverse-mode differentiation in FP64."""
+    W1, b1, W2, b2, W3, b3 = unpack(p)
+    X = np.asarray(X, np.float64)
+    n = X.shape[0]
+    z1 = X @ W1 + b1
+    h1 = silu(z1)
+    z2 = h1 @ W2 + b2
+    h2 = silu(z2)
+    out = h2 @ W3 + b3
+    e = out - np.asarray(Y, np.float64)
+    dy = 2.0 * e / n
+    gW3 = h2.T @ dy
+    gb3 = dy.sum(0)
+    d2 = (dy @ W3.T) * dsilu(z2)
+    gW2 = h1.T @ d2
+    gb2 = d2.sum(0)
+    d1 = (d2 @ W2.T) * dsilu(z1)
+    gW1 = X.T @ d1
+    gb1 = d1.sum(0)
+    return float(np.mean(np.sum(e * e, axis=1))), pack(gW1, gb1, gW2, gb2, gW3, gb3)
+
+
+def to_bf16(a) -> np.ndarray:
+    """Round to the nearest BF16 value, ties to even (returned as FP64)."""
+    u = np.asarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def forward_bf16(p, X) -> np.ndarray:
+    """The inference kernel's arithmetic: BF16 weights, inputs and hidden
+    activations; biases and accumulation in FP32 (here FP64)."""
+    W1, b1, W2, b2, W3, b3 = unpack(p)
+    b1, b2, b3 = (np.asarray(b, np.float32).astype(np.float64) for b in (b1, b2, b3))
+    h1 = to_bf16(silu(to_bf16(X) @ to_bf16(W1) + b1))
+    h2 = to_bf16(silu(h1 @ to_bf16(W2) + b2))
+    return h2 @ to_bf16(W3) + b3
+
+
+# ---- training tuples -------------------------------------------------------
+def node_features(x, com, mass, quad6, size, centre, bmax, count, eps=0.0) -> np.ndarray:
+    """Features of (target x, node). quad6 = traceless quadrupole xx xy xz yy
+    yz zz about the COM (sum m (3 y y^T - |y|^2 I)); centre = the node's cell
+    centre; size = the cell side; eps = the softening length."""
+    x = np.atleast_2d(np.asarray(x, np.float64))
+    r = x - np.asarray(com, np.float64)
+    f = np.zeros((x.shape[0], NF))
+    f[:, 0] = np.log2(max(int(count), 1)) / 16.0
+    q = np.asarray(quad6, np.float64) / (mass * size * size)
+    f[:, 1:7] = q[[0, 1, 2, 3, 4, 5]]
+    f[:, 7:10] = r / size
+    f[:, 10] = np.linalg.norm(r, axis=1) / size
+    f[:, 11:14] = (np.asarray(com, np.float64) - np.asarray(centre, np.float64)) / size
+    f[:, 14] = bmax / size
+    f[:, 15] = eps / size
+    return f
+
+
+def multipole_accel(x, com, mass, quad6, eps, trs=0.0) -> np.ndarray:
+    """Second-order expansion of the Plummer-softened force at x:
+    a = M d / R^3 - Q d / R^5 + 5/2 (d.Q d - eps^2 trs) d / R^7,
+    d = com - x, R^2 = |d|^2 + eps^2, trs = sum m |y|^2 about the COM."""
+    x = np.atleast_2d(np.asarray(x, np.float64))
+    d = np.asarray(com, np.float64) - x
+    r2 = np.sum(d * d, axis=1) + eps * eps
+    ri = 1.0 / np.sqrt(r2)
+    qxx, qxy, qxz, qyy, qyz, qzz = np.asarray(quad6, np.float64)
+    Q = np.array([[qxx, qxy, qxz], [qxy, qyy, qyz], [qxz, qyz, qzz]])
+    qd = d @ Q
+    dqd = np.sum(d * qd, axis=1)
+    return ((mass * ri**3)[:, None] * d - qd * (ri**5)[:, None]
+            + (2.5 * (dqd - eps * eps * trs) * ri**7)[:, None] * d)
+
+
+def direct_accel(x, pos, mass, eps) -> np.ndarray:
+    x = np.atleast_2d(np.asarray(x, np.float64))
+    d = np.asarray(pos, np.float64)[None, :, :] - x[:, None, :]
+    r2 = np.sum(d * d, axis=2) + eps * eps
+    w = np.asarray(mass, np.float64)[None, :] / (r2 * np.sqrt(r2))
+    return np.sum(d * w[:, :, None], axis=1)
+
+
+def node_moments(pos, mass):
+    """(M, com, traceless quadrupole[6], trs)."""
+    pos = np.asarray(pos, np.float64)
+    mass = np.asarray(mass, np.float64)
+    M = mass.sum()
+    com = (mass[:, None] * pos).sum(0) / M
+    y = pos - com
+    r2 = np.sum(y * y, axis=1)
+    q = [np.sum(mass * (3 * y[:, a] * y[:, b] - (r2 if a == b else 0.0)))
+         for a, b in ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))]
+    return M, com, np.array(q), float(np.sum(mass * r2))
+
+
+def node_residual(x, pos, mass, eps) -> np.ndarray:
+    """Training target: (direct - expansion) / (M / (|x - com|^2 + eps^2))."""
+    M, com, q, trs = node_moments(pos, mass)
+    x = np.atleast_2d(np.asarray(x, np.float64))
+    res = direct_accel(x, pos, mass, eps) - multipole_accel(x, com, M, q, eps, trs)
+    d2 = np.sum((x - com) ** 2, axis=1)
+    return res / (M / (d2 + eps * eps))[:, None]

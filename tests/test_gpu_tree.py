@@ -130,6 +130,10 @@ def test_moments_match_fp64(oracle):
         qref = ref[[4, 5, 6, 7, 8, 9]]
         scale = ref[0] * size * size
         assert np.abs(q - qref).max() <= 2e-4 * scale + 1e-12
+        # trace of the second moment (the softened expansion's eps^2 term)
+        y = pos[members] - ref[1:4]
+        trs = float(np.sum(m[members] * np.sum(y * y, axis=1)))
+        assert abs(mo[i, 11] - trs) <= 2e-4 * scale + 1e-12
         # bmax encloses every particle
         r = np.linalg.norm(pos[members] - ref[1:4], axis=1).max()
         assert mo[i, 9] >= r * (1 - 1e-5) - 1e-6 * size
