@@ -882,7 +882,7 @@ bool lff_alloc_core(bh_hip_handle* h)
                 "lff partial") ||
         !dalloc(h, &L.loss_partial, L.partial_blocks, "lff loss partial") ||
         !dalloc(h, &L.loss_acc, 4, "lff loss") || !dalloc(h, &L.bias, 144, "lff bias") ||
-        !dalloc(h, &L.tuple_counter, 4, "lff counter"))
+        !dalloc(h, &L.tuple_counter, 2, "lff counter"))
         return false;
     short* frag = nullptr;
     if (!dalloc(h, &frag, 24 * 64 * 4, "lff fragments"))
@@ -974,18 +974,35 @@ bh_hip_result_t bh_hip_lff_collect(bh_hip_handle_t* h, int32_t group_stride, uin
     bh::WalkParams wp = walk_params(h->p);
     wp.theta2 = h->lff_cfg.theta * h->lff_cfg.theta;
     bh::launch_walk_record(h->d, wp, h->stats, L, h->lff_cfg.band, h->lff_cfg.min_count);
+    // pass 1: count the candidate pairs
+    bh::launch_fill_u32(L.tuple_counter + 1, 0u, 1, h->d.stream);
     bh::launch_lff_tuples(h->d, L, group_stride, h->lff_cfg.band, wp.eps2, h->lff_cfg.min_count,
-                          static_cast<unsigned int>(L.tuple_cap));
-    const bh_hip_result_t r = finish(h, "lff_collect");
+                          0xffffffffu, L.collects, true, L.tuple_counter + 1, 0u);
+    bh_hip_result_t r = finish(h, "lff_collect");
+    unsigned int cnt[2] = {0, 0};
+    if (r == BH_HIP_OK &&
+        !check(h, hipMemcpy(cnt, L.tuple_counter, 8, hipMemcpyDeviceToHost), "lff count"))
+        r = BH_HIP_INTERNAL_ERROR;
+    if (r == BH_HIP_OK && cnt[0] < L.tuple_cap && cnt[1] > 0) {
+        // pass 2: a uniform hash subsample that fills (most of) the free room
+        const double room = static_cast<double>(L.tuple_cap - cnt[0]);
+        const double frac = std::min(1.0, 0.97 * room / static_cast<double>(cnt[1]));
+        const uint32_t keep =
+            frac >= 1.0 ? 0xffffffffu : static_cast<uint32_t>(frac * 4294967295.0);
+        bh::launch_lff_tuples(h->d, L, group_stride, h->lff_cfg.band, wp.eps2,
+                              h->lff_cfg.min_count, keep, L.collects, false, L.tuple_counter,
+                              static_cast<unsigned int>(L.tuple_cap));
+        r = finish(h, "lff_collect");
+    }
+    ++L.collects;
     end_call(h);
     h->primed = false;  // acc holds the uncorrected coarse walk
     if (r != BH_HIP_OK)
         return r;
-    unsigned int cnt = 0;
-    if (!check(h, hipMemcpy(&cnt, L.tuple_counter, 4, hipMemcpyDeviceToHost), "lff count"))
+    if (!check(h, hipMemcpy(cnt, L.tuple_counter, 4, hipMemcpyDeviceToHost), "lff count"))
         return BH_HIP_INTERNAL_ERROR;
     if (total)
-        *total = std::min<uint64_t>(cnt, L.tuple_cap);
+        *total = std::min<uint64_t>(cnt[0], L.tuple_cap);
     return BH_HIP_OK;
 }
 

@@ -187,8 +187,9 @@ struct LffState {
     float* bias = nullptr;        // b1[64], b2[64], b3[16]
     float* X = nullptr;           // training features, tuple_cap x 16
     float* Y = nullptr;           // training targets, tuple_cap x 3
-    unsigned int* tuple_counter = nullptr;
+    unsigned int* tuple_counter = nullptr;  // [0] tuples held, [1] scratch count
     size_t tuple_cap = 0;
+    uint32_t collects = 0;                  // collect calls so far (hash seed)
 };
 
 // ---- launchers (defined in the kernel translation units) -------------------
@@ -227,8 +228,11 @@ void launch_walk(Device& d, const WalkParams& p, unsigned long long* stats);
 void launch_walk_record(Device& d, const WalkParams& p, unsigned long long* stats,
                         const LffState& L, float band, int min_count);
 // k_lff.hip
+// Training tuples of every group_stride-th group; a pair is kept when its
+// hash is <= keep. count_only: only count the kept pairs into *counter.
 void launch_lff_tuples(Device& d, const LffState& L, int group_stride, float band, float eps2,
-                       int min_count, unsigned int max_tuples);
+                       int min_count, uint32_t keep, uint32_t seed, bool count_only,
+                       unsigned int* counter, unsigned int max_tuples);
 void launch_lff_grad(Device& d, const LffState& L, const float* X, const float* Y,
                      const uint32_t* order, size_t first, uint32_t nbatch, float yscale,
                      const float* P, int nblocks);

@@ -12,7 +12,7 @@
 // with softening eps:
 //   f0 = log2(n)/16, f1..f6 = Q/(M s^2) (xx xy xz yy yz zz),
 //   f7..f9 = (x - c)/s, f10 = |x - c|/s, f11..f13 = (c - g)/s, f14 = bmax/s,
-//   f15 = eps/s.
+//   f15 = eps^2 / (|x - c|^2 + eps^2).
 // The residual is taken against the walk's second-order softened expansion
 // (monopole, traceless quadrupole and the eps^2 trs term). Only nodes of at
 // least min_count (>= 2) particles are corrected: below a few tens of
@@ -60,6 +60,9 @@ static_assert(OFF_B3 + NO == NP, "parameter layout");
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+typedef __bf16 bf16v4 __attribute__((ext_vector_type(4)));
+
+// Training arithmetic (FP32, correctly rounded division).
 __device__ inline float sigmoidf(float z) { return 1.0f / (1.0f + __expf(-z)); }
 __device__ inline float siluf(float z) { return z * sigmoidf(z); }
 __device__ inline float dsiluf(float z)
@@ -67,49 +70,74 @@ __device__ inline float dsiluf(float z)
     const float s = sigmoidf(z);
     return s * (1.0f + z * (1.0f - s));
 }
-
-// FP32 -> BF16 bits, round to nearest even.
-__device__ inline short f2bf(float f)
+// Inference activation: one v_exp_f32 and one v_rcp_f32.
+__device__ inline float silu_fast(float z)
 {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return static_cast<short>(u >> 16);
+    return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z));
 }
 
+// FP32 -> BF16, round to nearest even (v_cvt_pk_bf16_f32).
 __device__ inline bf16x4 pack4(float a, float b, float c, float d)
 {
-    bf16x4 v;
-    v[0] = f2bf(a);
-    v[1] = f2bf(b);
-    v[2] = f2bf(c);
-    v[3] = f2bf(d);
-    return v;
+    bf16v4 v;
+    v[0] = static_cast<__bf16>(a);
+    v[1] = static_cast<__bf16>(b);
+    v[2] = static_cast<__bf16>(c);
+    v[3] = static_cast<__bf16>(d);
+    return __builtin_bit_cast(bf16x4, v);
 }
 
-// Features of one (target, node) pair with r = x - c; s = the node's cell
-// size, cnt = its particle count, eps = the softening length.
-__device__ inline float lff_feature(const WalkNode& w, float s, uint32_t cnt, float eps, float rx,
-                                    float ry, float rz, int k)
+__device__ inline uint32_t mix32(uint32_t x)
 {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// Node part of the features, shared by every target of the node (s = the
+// cell size, cnt = the particle count).
+struct NodeFeat {
+    float f0, q[6], c[3], b, inv_s;
+};
+
+__device__ inline NodeFeat node_feat(const WalkNode& w, float s, uint32_t cnt)
+{
+    NodeFeat n;
+    n.inv_s = 1.0f / s;
+    n.f0 = log2f(static_cast<float>(cnt > 0 ? cnt : 1)) * (1.0f / 16.0f);
     const float qn = w.mass > 0.0f ? 1.0f / (w.mass * s * s) : 0.0f;
-    switch (k) {
-    case 0: return log2f(static_cast<float>(cnt > 0 ? cnt : 1)) * (1.0f / 16.0f);
-    case 1: return w.qxx * qn;
-    case 2: return w.qxy * qn;
-    case 3: return w.qxz * qn;
-    case 4: return w.qyy * qn;
-    case 5: return w.qyz * qn;
-    case 6: return (-w.qxx - w.qyy) * qn;
-    case 7: return rx / s;
-    case 8: return ry / s;
-    case 9: return rz / s;
-    case 10: return sqrtf(rx * rx + ry * ry + rz * rz) / s;
-    case 11: return w.comx / s;
-    case 12: return w.comy / s;
-    case 13: return w.comz / s;
-    case 14: return w.bmax / s;
-    default: return eps / s;
-    }
+    n.q[0] = w.qxx * qn;
+    n.q[1] = w.qxy * qn;
+    n.q[2] = w.qxz * qn;
+    n.q[3] = w.qyy * qn;
+    n.q[4] = w.qyz * qn;
+    n.q[5] = (-w.qxx - w.qyy) * qn;
+    n.c[0] = w.comx * n.inv_s;
+    n.c[1] = w.comy * n.inv_s;
+    n.c[2] = w.comz * n.inv_s;
+    n.b = w.bmax * n.inv_s;
+    return n;
+}
+
+// All 16 features of (node, target at r = x - c), d2 = |r|^2.
+__device__ inline void pair_features(const NodeFeat& n, float rx, float ry, float rz, float d2,
+                                     float eps2, float F[NF])
+{
+    F[0] = n.f0;
+    for (int k = 0; k < 6; ++k)
+        F[1 + k] = n.q[k];
+    F[7] = rx * n.inv_s;
+    F[8] = ry * n.inv_s;
+    F[9] = rz * n.inv_s;
+    F[10] = sqrtf(d2) * n.inv_s;
+    F[11] = n.c[0];
+    F[12] = n.c[1];
+    F[13] = n.c[2];
+    F[14] = n.b;
+    F[15] = d2 + eps2 > 0.0f ? eps2 / (d2 + eps2) : 0.0f;
 }
 
 // Second-order softened expansion of node w at offset d = c - x (the walk's
@@ -155,21 +183,24 @@ __device__ inline void group_targets(const uint4& gr, const WalkNode* __restrict
 }
 
 // ---- training tuples -------------------------------------------------------
+// Every group_stride-th group; each (target, recorded node) pair in the band
+// is kept when its hash falls at or below keep (a uniform subsample). With
+// X == nullptr the kernel only counts the pairs (no direct sums).
 __global__ __launch_bounds__(64) void lff_tuple_kernel(
     const WalkNode* __restrict__ nodes, const uint32_t* __restrict__ nstart,
     const uint32_t* __restrict__ ncount, const float4* __restrict__ spart,
     const uint32_t* __restrict__ sleaf, const RootGeom* __restrict__ geom,
     const uint4* __restrict__ groups, const int* __restrict__ info,
     const uint32_t* __restrict__ lists, const uint32_t* __restrict__ list_n, int list_cap,
-    int list_groups, int group_stride, float band, float eps2, uint32_t min_count,
-    float* __restrict__ X,
-    float* __restrict__ Y, unsigned int* __restrict__ counter, unsigned int max_tuples)
+    int list_groups, int group_stride, float band, float eps2, uint32_t min_count, uint32_t keep,
+    uint32_t seed, float* __restrict__ X, float* __restrict__ Y,
+    unsigned int* __restrict__ counter, unsigned int max_tuples)
 {
     const int ngroups = min(info[kInfoGroups], list_groups);
     const float ux = geom->unit_f[0], uy = geom->unit_f[1], uz = geom->unit_f[2];
     const float wfmax = fmaxf(fmaxf(geom->width_f[0], geom->width_f[1]), geom->width_f[2]);
-    const float eps = sqrtf(eps2);
     const int lane = threadIdx.x;
+    const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     for (int g = blockIdx.x * group_stride; g < ngroups; g += gridDim.x * group_stride) {
         const uint4 gr = groups[g];
         int rcx, rcy, rcz;
@@ -187,6 +218,21 @@ __global__ __launch_bounds__(64) void lff_tuple_kernel(
             const float cx = rel_pos(w.cx - rcx, ux, w.comx);
             const float cy = rel_pos(w.cy - rcy, uy, w.comy);
             const float cz = rel_pos(w.cz - rcz, uz, w.comz);
+            const float rx = tx - cx, ry = ty - cy, rz = tz - cz;
+            const float d2 = rx * rx + ry * ry + rz * rz;
+            const uint32_t h = mix32(static_cast<uint32_t>(g) * 0x9e3779b1u ^
+                                     mix32(static_cast<uint32_t>(node) + seed) ^
+                                     (static_cast<uint32_t>(lane) << 24));
+            const bool take = active && size * size >= band * band * d2 && h <= keep;
+            const unsigned long long tm = __ballot(take);
+            if (tm == 0ull)
+                continue;
+            unsigned int base = 0;
+            if (lane == 0)
+                base = atomicAdd(counter, static_cast<unsigned int>(__popcll(tm)));
+            base = __shfl(base, 0, 64);
+            if (!X)
+                continue;
             float ex = 0.0f, ey = 0.0f, ez = 0.0f;
             for (uint32_t j = a; j < a + cnt; ++j) {
                 const float4 q = spart[j];
@@ -202,22 +248,18 @@ __global__ __launch_bounds__(64) void lff_tuple_kernel(
                 ey += sc * dy;
                 ez += sc * dz;
             }
-            float qx, qy, qz;
-            quad_accel(w, cx - tx, cy - ty, cz - tz, eps2, qx, qy, qz);
-            const float rx = tx - cx, ry = ty - cy, rz = tz - cz;
-            const float d2 = rx * rx + ry * ry + rz * rz;
-            const bool take = active && size * size >= band * band * d2;
-            if (take) {
-                const unsigned int idx = atomicAdd(counter, 1u);
-                if (idx < max_tuples) {
-                    const float mono = w.mass / (d2 + eps2);
-                    for (int f = 0; f < NF; ++f)
-                        X[static_cast<size_t>(idx) * NF + f] =
-                            lff_feature(w, size, cnt, eps, rx, ry, rz, f);
-                    Y[static_cast<size_t>(idx) * NO + 0] = (ex - qx) / mono;
-                    Y[static_cast<size_t>(idx) * NO + 1] = (ey - qy) / mono;
-                    Y[static_cast<size_t>(idx) * NO + 2] = (ez - qz) / mono;
-                }
+            const unsigned int idx = base + static_cast<unsigned int>(__popcll(tm & lt));
+            if (take && idx < max_tuples) {
+                float qx, qy, qz;
+                quad_accel(w, cx - tx, cy - ty, cz - tz, eps2, qx, qy, qz);
+                const float mono = w.mass / (d2 + eps2);
+                float F[NF];
+                pair_features(node_feat(w, size, cnt), rx, ry, rz, d2, eps2, F);
+                for (int f = 0; f < NF; ++f)
+                    X[static_cast<size_t>(idx) * NF + f] = F[f];
+                Y[static_cast<size_t>(idx) * NO + 0] = (ex - qx) / mono;
+                Y[static_cast<size_t>(idx) * NO + 1] = (ey - qy) / mono;
+                Y[static_cast<size_t>(idx) * NO + 2] = (ez - qz) / mono;
             }
         }
     }
@@ -449,14 +491,14 @@ __device__ inline f32x4 mlp_mfma(const MlpRegs& R, const bf16x4 x)
     bf16x4 h[4];
     for (int mt = 0; mt < 4; ++mt) {
         const f32x4 c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(R.w1[mt], x, R.b1[mt], 0, 0, 0);
-        h[mt] = pack4(siluf(c[0]), siluf(c[1]), siluf(c[2]), siluf(c[3]));
+        h[mt] = pack4(silu_fast(c[0]), silu_fast(c[1]), silu_fast(c[2]), silu_fast(c[3]));
     }
     bf16x4 h2[4];
     for (int mt = 0; mt < 4; ++mt) {
         f32x4 c = R.b2[mt];
         for (int kt = 0; kt < 4; ++kt)
             c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(R.w2[4 * mt + kt], h[kt], c, 0, 0, 0);
-        h2[mt] = pack4(siluf(c[0]), siluf(c[1]), siluf(c[2]), siluf(c[3]));
+        h2[mt] = pack4(silu_fast(c[0]), silu_fast(c[1]), silu_fast(c[2]), silu_fast(c[3]));
     }
     f32x4 c = R.b3;
     for (int kt = 0; kt < 4; ++kt)
@@ -501,7 +543,7 @@ __global__ __launch_bounds__(64) void lff_apply_kernel(
     const int lane = threadIdx.x, r = lane & 15, q = lane >> 4;
     const float ux = geom->unit_f[0], uy = geom->unit_f[1], uz = geom->unit_f[2];
     const float wfmax = fmaxf(fmaxf(geom->width_f[0], geom->width_f[1]), geom->width_f[2]);
-    const float eps = sqrtf(eps2);
+    const float band2 = band * band;
     MlpRegs R;
     load_mlp(R, frag, bias, lane);
     const int ngroups = info[kInfoGroups];
@@ -528,22 +570,28 @@ __global__ __launch_bounds__(64) void lff_apply_kernel(
             const float ncx = rel_pos(w.cx - rcx, ux, w.comx);
             const float ncy = rel_pos(w.cy - rcy, uy, w.comy);
             const float ncz = rel_pos(w.cz - rcz, uz, w.comz);
+            const NodeFeat nf = node_feat(w, size, cnt);
+#pragma unroll
             for (int ct = 0; ct < 4; ++ct) {
-                if (16 * ct >= nact)
-                    break;  // uniform across the wave
-                const int p = 16 * ct + r;
-                const float rx = sT[p][0] - ncx, ry = sT[p][1] - ncy, rz = sT[p][2] - ncz;
-                const f32x4 c =
-                    mlp_mfma(R, pack4(lff_feature(w, size, cnt, eps, rx, ry, rz, 4 * q),
-                                      lff_feature(w, size, cnt, eps, rx, ry, rz, 4 * q + 1),
-                                      lff_feature(w, size, cnt, eps, rx, ry, rz, 4 * q + 2),
-                                      lff_feature(w, size, cnt, eps, rx, ry, rz, 4 * q + 3)));
-                const float d2 = rx * rx + ry * ry + rz * rz;
-                if (q == 0 && p < nact && size * size >= band * band * d2) {
-                    const float mono = w.mass / (d2 + eps2);
-                    cx[ct] += c[0] * mono;
-                    cy[ct] += c[1] * mono;
-                    cz[ct] += c[2] * mono;
+                if (16 * ct < nact) {  // uniform across the wave
+                    const int p = 16 * ct + r;
+                    const float rx = sT[p][0] - ncx, ry = sT[p][1] - ncy, rz = sT[p][2] - ncz;
+                    const float d2 = rx * rx + ry * ry + rz * rz;
+                    const bool in = p < nact && size * size >= band2 * d2;
+                    if (__ballot(in) != 0ull) {  // uniform: skip tiles with no pair in the band
+                        float F[NF];
+                        pair_features(nf, rx, ry, rz, d2, eps2, F);
+                        float v[4];
+                        for (int j = 0; j < 4; ++j)  // lane quarter q holds features 4q..4q+3
+                            v[j] = q == 0 ? F[j] : q == 1 ? F[4 + j] : q == 2 ? F[8 + j] : F[12 + j];
+                        const f32x4 c = mlp_mfma(R, pack4(v[0], v[1], v[2], v[3]));
+                        if (q == 0 && in) {
+                            const float mono = w.mass / (d2 + eps2);
+                            cx[ct] += c[0] * mono;
+                            cy[ct] += c[1] * mono;
+                            cz[ct] += c[2] * mono;
+                        }
+                    }
                 }
             }
         }
@@ -572,14 +620,15 @@ __global__ __launch_bounds__(64) void lff_apply_kernel(
 }  // namespace
 
 void launch_lff_tuples(Device& d, const LffState& L, int group_stride, float band, float eps2,
-                       int min_count, unsigned int max_tuples)
+                       int min_count, uint32_t keep, uint32_t seed, bool count_only,
+                       unsigned int* counter, unsigned int max_tuples)
 {
     const unsigned grid = static_cast<unsigned>(std::max<size_t>(1, d.stack_waves));
     lff_tuple_kernel<<<grid, 64, 0, d.stream>>>(
         d.wnode, d.nstart, d.ncount, d.spart, d.sleaf, d.geom, d.groups, d.level_info, L.lists,
         L.list_n, L.list_cap, static_cast<int>(L.list_groups), group_stride, band, eps2,
-        static_cast<uint32_t>(std::max(2, min_count)), L.X, L.Y,
-        L.tuple_counter, max_tuples);
+        static_cast<uint32_t>(std::max(2, min_count)), keep, seed, count_only ? nullptr : L.X,
+        L.Y, counter, max_tuples);
 }
 
 void launch_lff_grad(Device& d, const LffState& L, const float* X, const float* Y,

@@ -133,7 +133,8 @@ def node_features(x, com, mass, quad6, size, centre, bmax, count, eps=0.0) -> np
     f[:, 10] = np.linalg.norm(r, axis=1) / size
     f[:, 11:14] = (np.asarray(com, np.float64) - np.asarray(centre, np.float64)) / size
     f[:, 14] = bmax / size
-    f[:, 15] = eps / size
+    d2 = np.sum(r * r, axis=1)
+    f[:, 15] = np.where(d2 + eps * eps > 0, eps * eps / np.maximum(d2 + eps * eps, 1e-300), 0.0)
     return f
 
 

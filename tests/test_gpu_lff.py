@@ -116,7 +116,7 @@ def test_tuples_match_fp64_oracle():
         t = s.dump_tree()
     assert total == len(X) > 1000
     assert np.all(np.isfinite(X)) and np.all(np.isfinite(Y))
-    assert np.all(X[:, 15] > 0.0)  # eps / s
+    assert np.all((X[:, 15] > 0.0) & (X[:, 15] <= 1.0))  # eps^2 / (d^2 + eps^2)
     assert np.all(X[:, 10] <= 1.0 / band * (1 + 1e-5))  # size >= band * distance
     g = t["geom"]
     lo, unit = g[6:9], g[9:12]

@@ -138,7 +138,8 @@ def test_features_and_residual_invariances():
     np.testing.assert_allclose(f[:, 10], np.linalg.norm(f[:, 7:10], axis=1), rtol=1e-14)
     np.testing.assert_allclose(f[:, 6], -(f[:, 1] + f[:, 4]), rtol=1e-12, atol=1e-15)
     assert f[0, 0] == pytest.approx(np.log2(25) / 16)
-    assert np.all(f[:, 15] == 0.02)
+    d2 = np.sum((x - com) ** 2, axis=1)
+    np.testing.assert_allclose(f[:, 15], 0.02 ** 2 / (d2 + 0.02 ** 2), rtol=1e-14)
 
 
 @pytest.mark.parametrize("eps", [0.0, 0.3, 1.0, 4.0])
