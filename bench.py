@@ -99,6 +99,29 @@ def force_error(acc, ref, samples):
             "max": float(r.max()), "samples": int(samples), "sample_seed": 2024}
 
 
+def block_time_steps(s, args, n):
+    """Dyadic block steps on the same state: theta 0.5, dt_i = 0.02 sqrt(eps /
+    |a_i|) clipped to [2^-14, 2^-6] (base step 2^-6, 8 levels, no velocity
+    bound). value = particle steps (one per particle per own step) per second."""
+    s.set_params(theta=args.ref_theta, adaptive_max_level=8, adaptive_eta=0.02,
+                 adaptive_flags=0)
+    s.step(2.0 ** -6, 1)  # initialises the schedule, one warm-up base step
+    t0 = time.perf_counter()
+    s.step(2.0 ** -6, args.block_steps)
+    el = time.perf_counter() - t0
+    tm = s.timings()
+    levels = np.bincount(s.adaptive_levels(), minlength=9)
+    slices = max(1, tm["walks"])
+    out = {"base_dt": 2.0 ** -6, "levels": 8, "eta": 0.02, "theta": args.ref_theta,
+           "base_steps": args.block_steps, "slices": tm["walks"],
+           "particle_steps": tm["particle_steps"], "value": tm["particle_steps"] / el,
+           "unit": "particle-steps/s", "ms_per_base_step": el * 1e3 / args.block_steps,
+           "build_ms_per_slice": tm["build_ms"] / slices, "walk_ms_per_slice": tm["walk_ms"] / slices,
+           "level_histogram": levels.tolist()}
+    s.set_params(adaptive_max_level=0)
+    return out
+
+
 def learned_far_field(s, args, n, m_now, t, with_error):
     """theta 0.9 walk with the learned correction: trained on a second
     realisation (seed 12) of the workload, applied to the timed state."""
@@ -146,6 +169,8 @@ def main(argv=None):
     ap.add_argument("--ref-theta", type=float, default=0.5)
     ap.add_argument("--ref-steps", type=int, default=5)
     ap.add_argument("--lff", action="store_true")
+    ap.add_argument("--block-steps", type=int, default=0,
+                    help="also run N base steps with dyadic block time steps")
     ap.add_argument("--eps", type=float, default=0.01)
     ap.add_argument("--order", type=int, default=2)
     ap.add_argument("--bucket", type=int, default=16)
@@ -216,6 +241,11 @@ def main(argv=None):
                                               s.direct_forces_fp64(t), args.samples)
     log(f"[bench] theta {args.ref_theta}: {ref_line}")
 
+    block_line = None
+    if args.block_steps > 0:
+        block_line = block_time_steps(s, args, n)
+        log(f"[bench] block time steps: {block_line}")
+
     lff_line = None
     if args.lff:
         lff_line = learned_far_field(s, args, n, m, t, ref is not None)
@@ -266,6 +296,8 @@ def main(argv=None):
     }
     if lff_line is not None:
         line["learned_far_field"] = lff_line
+    if block_line is not None:
+        line["block_time_steps"] = block_line
     print(json.dumps(line), flush=True)
     return 0
 

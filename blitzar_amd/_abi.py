@@ -36,7 +36,10 @@ class Params(C.Structure):
         ("treepm_split_scale", C.c_float),
         ("treepm_cutoff", C.c_float),
         ("waves_per_cu", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("adaptive_max_level", C.c_int32),
+        ("adaptive_eta", C.c_float),
+        ("adaptive_flags", C.c_int32),
+        ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -53,6 +56,7 @@ class Timings(C.Structure):
         ("max_level", C.c_int32),
         ("walks", C.c_int32),
         ("groups", C.c_int32),
+        ("particle_steps", C.c_uint64),
     ]
 
 
@@ -81,7 +85,7 @@ EXPORTS = [
     "bh_hip_get_timings", "bh_hip_get_last_error", "bh_hip_sort_pairs",
     "bh_hip_lff_configure", "bh_hip_lff_collect", "bh_hip_lff_train", "bh_hip_lff_get_params",
     "bh_hip_lff_set_params", "bh_hip_lff_gradient", "bh_hip_lff_forward",
-    "bh_hip_lff_get_tuples",
+    "bh_hip_lff_get_tuples", "bh_hip_adaptive_levels",
     "bh_ic_plummer", "bh_ic_hernquist", "bh_ic_hernquist_merger", "bh_ic_uniform_cube",
 ]
 
@@ -169,6 +173,8 @@ def lib() -> C.CDLL:
     L.bh_hip_lff_forward.argtypes = [H, _p(np.float32), C.c_uint32, _p(np.float32)]
     L.bh_hip_lff_get_tuples.argtypes = [H, _opt(np.float32), _opt(np.float32), C.c_uint64,
                                         C.POINTER(C.c_uint64)]
+    L.bh_hip_adaptive_levels.argtypes = [H, _p(np.int32)]
+    L.bh_hip_adaptive_levels.restype = R
     for name in ("bh_hip_lff_configure", "bh_hip_lff_collect", "bh_hip_lff_train",
                  "bh_hip_lff_get_params", "bh_hip_lff_set_params", "bh_hip_lff_gradient",
                  "bh_hip_lff_forward", "bh_hip_lff_get_tuples"):

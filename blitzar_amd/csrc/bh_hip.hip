@@ -35,11 +35,14 @@ struct bh_hip_handle {
     int leaf_depth = 1;
     int cu_count = 256;
     bool primed = false;      // acc holds a(x) of the current positions
+    bool ad_primed = false;   // block time-step schedule initialised
+    double ad_quantum = 0.0;  // slice length of the schedule
+    uint64_t ad_tick = 0;     // slices since the schedule started
     bool have_tree = false;   // the last build matches the current positions
     bool rank_valid = true;   // rank (caller index -> slot) matches the store order
     bool timing = false;
     std::string last_error;
-    unsigned long long* stats = nullptr;  // device: pp, pc
+    unsigned long long* stats = nullptr;  // device: pp, pc, block-step particle steps
     std::vector<StepEvents> events;
     size_t events_used = 0;
     double integrate_ms = 0.0;
@@ -106,9 +109,13 @@ bh_hip_params_t resolve(const bh_hip_params_t& in)
     p.theta = std::clamp(in.theta, min_theta, 4.0f);
     p.bucket_capacity = std::max(1, in.bucket_capacity);
     p.multipole_order = in.multipole_order >= 2 ? 2 : 1;
-    p.criterion = in.criterion == 1 ? 1 : 0;
+    p.criterion = in.criterion == 1 || in.criterion == 2 ? in.criterion : 0;
     p.key_mode = in.key_mode == 1 ? 1 : 0;
     p.integrator = in.integrator == 1 ? 1 : 0;
+    // CfgScenarioRuntime.cpp:63-68: max level <= 12, eta in [0.01, 1]
+    p.adaptive_max_level = std::clamp(in.adaptive_max_level, 0, 12);
+    p.adaptive_eta = std::clamp(in.adaptive_eta, 0.01f, 1.0f);
+    p.adaptive_flags = in.adaptive_flags & BH_HIP_ADAPTIVE_VELOCITY;
     if (p.waves_per_cu <= 0)
         p.waves_per_cu = kDefaultWavesPerCu;
     return p;
@@ -179,7 +186,7 @@ bool alloc_all(bh_hip_handle* h, uint64_t n)
     if (!dalloc(h, &d.level_info, 2 * bh::kLevels + 8, "level info") ||
         !dalloc(h, &d.geom, 1, "geometry") || !dalloc(h, &d.aabb_bits, 8, "aabb") ||
         !dalloc(h, &d.err, 4, "error flags") || !dalloc(h, &d.work_counter, 4, "work counter") ||
-        !dalloc(h, &h->stats, 2, "walk stats"))
+        !dalloc(h, &h->stats, 3, "walk stats"))
         return false;
     d.diag_cap = 6 * 1024 + 8 * 256 + 16;  // AABB partials, energy partials
     if (!dalloc(h, &d.diag, d.diag_cap, "diagnostics"))
@@ -274,6 +281,8 @@ void ensure_rank(bh_hip_handle* h)
     }
 }
 
+
+
 bh::WalkParams walk_params(const bh_hip_params_t& p)
 {
     bh::WalkParams w{};
@@ -323,11 +332,12 @@ void queue_force(bh_hip_handle* h)
     h->primed = true;
 }
 
+
 void begin_call(bh_hip_handle* h)
 {
     h->events_used = 0;
     h->integrate_ms = 0.0;
-    bh::launch_fill_u32(reinterpret_cast<uint32_t*>(h->stats), 0u, 4, h->d.stream);
+    bh::launch_fill_u32(reinterpret_cast<uint32_t*>(h->stats), 0u, 6, h->d.stream);
 }
 
 // After a synchronised call: fold event times and counters into h->t.
@@ -347,10 +357,11 @@ void end_call(bh_hip_handle* h)
         }
         t.integrate_ms = h->integrate_ms;
     }
-    unsigned long long st[2] = {0, 0};
+    unsigned long long st[3] = {0, 0, 0};
     if (hipMemcpy(st, h->stats, sizeof(st), hipMemcpyDeviceToHost) == hipSuccess) {
         t.pp_interactions = st[0];
         t.pc_interactions = st[1];
+        t.particle_steps = st[2];
     }
     int info[2 * bh::kLevels + 4];
     if (hipMemcpy(info, h->d.level_info, sizeof(info), hipMemcpyDeviceToHost) == hipSuccess) {
@@ -361,6 +372,98 @@ void end_call(bh_hip_handle* h)
     t.leaf_depth = h->leaf_depth;
     t.walks = static_cast<int32_t>(h->events_used);
     h->t = t;
+}
+
+// Block time steps (engine/physics/core/cuda/integration/CudOctreeGpuAdaptive.inl:
+// 80-117 for the schedule, CudAdaptive.inl for the per-particle rules): each
+// global step of dt runs 2^L slices of q = dt / 2^L; every slice drifts all
+// particles with their last force, rebuilds the tree at the predicted
+// positions, walks the groups that hold a particle due a force, and corrects
+// those particles. (The reference GPU path keeps the tree of the start of the
+// global step for all slices and corrects a particle while other threads may
+// read it; here the tree is rebuilt per slice and the correction runs after
+// the walk.)
+bh::AdaptiveParams adaptive_params(const bh_hip_handle* h, double dt)
+{
+    bh::AdaptiveParams a;
+    a.levels = std::min(h->p.adaptive_max_level, 12);
+    a.eta = h->p.adaptive_eta;
+    a.soft = std::max(h->p.softening, h->p.min_softening);
+    a.dt = dt;
+    a.quantum = dt / static_cast<double>(1u << a.levels);
+    a.velocity = (h->p.adaptive_flags & BH_HIP_ADAPTIVE_VELOCITY) != 0;
+    a.max_speed = h->p.max_speed;
+    return a;
+}
+
+bool alloc_adaptive(bh_hip_handle* h)
+{
+    bh::Device& d = h->d;
+    if (d.alevel)
+        return true;
+    for (int a = 0; a < 3; ++a)
+        if (!dalloc(h, &d.aacc[a], d.n, "adaptive acc") ||
+            !dalloc(h, &d.aacc_b[a], d.n, "adaptive acc"))
+            return false;
+    uint32_t *lv = nullptr, *lt = nullptr, *lvb = nullptr, *ltb = nullptr;
+    if (!dalloc(h, &lv, d.n, "adaptive level") || !dalloc(h, &lt, d.n, "adaptive tick") ||
+        !dalloc(h, &lvb, d.n, "adaptive level") || !dalloc(h, &ltb, d.n, "adaptive tick"))
+        return false;
+    bh::launch_fill_u32(lv, 0u, d.n, d.stream);
+    bh::launch_fill_u32(lt, 0u, d.n, d.stream);
+    d.atick = lt;
+    d.alevel_b = lvb;
+    d.atick_b = ltb;
+    d.alevel = lv;  // last: its presence switches the permutation on
+    return true;
+}
+
+bh_hip_result_t step_adaptive(bh_hip_handle* h, double dt, int32_t nsteps)
+{
+    if (lff_ready(h)) {
+        h->last_error = "step: block time steps do not combine with the learned correction";
+        return BH_HIP_INVALID_ARGUMENT;
+    }
+    if (!alloc_adaptive(h))
+        return BH_HIP_INTERNAL_ERROR;
+    begin_call(h);
+    bh::Device& d = h->d;
+    const bh::AdaptiveParams a = adaptive_params(h, dt);
+    const bh::WalkParams wp0 = walk_params(h->p);
+    if (!h->ad_primed || std::fabs(h->ad_quantum - a.quantum) > 1e-12 * a.quantum) {
+        StepEvents* ev = next_events(h);
+        queue_build(h, ev);
+        bh::launch_walk(d, wp0, h->stats);
+        rec(ev ? ev->w1 : nullptr, d.stream);
+        bh::launch_adaptive_init(d, a, static_cast<uint32_t>(h->ad_tick));
+        h->ad_primed = true;
+        h->ad_quantum = a.quantum;
+    }
+    const uint64_t slices = 1ull << a.levels;
+    for (int k = 0; k < nsteps; ++k) {
+        for (uint64_t sl = 0; sl < slices; ++sl) {
+            const uint32_t t = static_cast<uint32_t>(++h->ad_tick);
+            bh::launch_adaptive_drift(d, a);
+            StepEvents* ev = next_events(h);
+            queue_build(h, ev);
+            bh::WalkParams wp = wp0;
+            wp.alevel = d.alevel;
+            wp.atick = t;
+            wp.alevels = a.levels;
+            bh::launch_walk(d, wp, h->stats);
+            rec(ev ? ev->w1 : nullptr, d.stream);
+            bh::launch_adaptive_correct(d, a, t, h->stats + 2);
+        }
+        // keep the host queue bounded on long runs
+        if (!check(h, hipStreamSynchronize(d.stream), "adaptive step")) {
+            end_call(h);
+            return BH_HIP_INTERNAL_ERROR;
+        }
+    }
+    h->primed = false;
+    const bh_hip_result_t r = finish(h, "step (block time steps)");
+    end_call(h);
+    return r;
 }
 
 bool stage_to_device(bh_hip_handle* h, const double* src3)
@@ -394,6 +497,9 @@ void bh_hip_default_params(bh_hip_params_t* out)
     out->key_mode = BH_HIP_KEYS_MORTON63;
     out->integrator = BH_HIP_INTEGRATOR_LEAPFROG;
     out->waves_per_cu = kDefaultWavesPerCu;
+    out->adaptive_max_level = 0;  // off (SrvSimulationServer.hpp:412-414: max level 4 when on)
+    out->adaptive_eta = 0.25f;
+    out->adaptive_flags = BH_HIP_ADAPTIVE_VELOCITY;
 }
 
 const char* bh_hip_version(void) { return "octree_hip 0.1 (gfx950)"; }
@@ -475,6 +581,9 @@ bh_hip_result_t bh_hip_set_params(bh_hip_handle_t* h, const bh_hip_params_t* par
             return BH_HIP_INVALID_ARGUMENT;
         }
     }
+    if (next.adaptive_max_level != old.adaptive_max_level ||
+        next.adaptive_eta != old.adaptive_eta || next.adaptive_flags != old.adaptive_flags)
+        h->ad_primed = false;
     h->p = next;
     h->p.waves_per_cu = old.waves_per_cu;  // residency is fixed at creation
     h->leaf_depth = reference_leaf_depth(h->d.n, h->p.leaf_capacity);
@@ -524,6 +633,7 @@ bh_hip_result_t bh_hip_upload(bh_hip_handle_t* h, const double* pos3, const doub
         }
     }
     h->primed = false;
+    h->ad_primed = false;
     h->have_tree = false;
     if (!ok)
         return BH_HIP_INTERNAL_ERROR;
@@ -603,6 +713,8 @@ bh_hip_result_t bh_hip_step(bh_hip_handle_t* h, double dt, int32_t nsteps)
     }
     if (!zero_err(h))
         return BH_HIP_INTERNAL_ERROR;
+    if (h->p.adaptive_max_level > 0)
+        return step_adaptive(h, dt, nsteps);
     begin_call(h);
     bh::Device& d = h->d;
     const double vmax = h->p.max_speed;
@@ -1208,3 +1320,23 @@ bh_hip_result_t bh_hip_lff_get_tuples(bh_hip_handle_t* h, float* X, float* Y, ui
 }
 
 }  // extern "C"
+
+extern "C" bh_hip_result_t bh_hip_adaptive_levels(bh_hip_handle_t* h, int32_t* levels)
+{
+    if (!h || !levels)
+        return BH_HIP_INVALID_ARGUMENT;
+    bh::Device& d = h->d;
+    if (!d.alevel) {
+        std::memset(levels, 0, d.n * sizeof(int32_t));
+        return BH_HIP_OK;
+    }
+    ensure_rank(h);
+    uint32_t* tmp = reinterpret_cast<uint32_t*>(d.stage);
+    bh::launch_gather_u32(d, d.alevel, tmp);
+    if (!check(h, hipGetLastError(), "adaptive_levels") ||
+        !check(h, hipStreamSynchronize(d.stream), "adaptive_levels") ||
+        !check(h, hipMemcpy(levels, tmp, d.n * sizeof(uint32_t), hipMemcpyDeviceToHost),
+               "adaptive_levels"))
+        return BH_HIP_INTERNAL_ERROR;
+    return BH_HIP_OK;
+}

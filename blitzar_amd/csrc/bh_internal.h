@@ -108,6 +108,23 @@ struct WalkParams {
     int order;          // 1 = monopole, 2 = quadrupole
     float rs;           // TreePM split scale (0 = Newton)
     float rcut2;        // TreePM cutoff squared (0 = none)
+    // Block time steps: when alevel is set, only groups holding a particle
+    // active at tick `atick` are walked (active: atick mod 2^(alevels -
+    // level) == 0); the other groups keep their previous acc.
+    const uint32_t* alevel = nullptr;
+    uint32_t atick = 0;
+    int alevels = 0;
+};
+
+// Block time-step settings of one adaptive slice.
+struct AdaptiveParams {
+    int levels;       // L = min(max_level, 12)
+    double eta;
+    double soft;      // max(softening, min_softening)
+    double dt;        // global step
+    double quantum;   // dt / 2^L
+    int velocity;     // also bound by eta * soft / |v|
+    double max_speed;
 };
 
 // Device buffers of one solver instance. The particle store is kept in the
@@ -163,6 +180,16 @@ struct Device {
     int* stack = nullptr;  // per-resident-wave traversal stacks
     size_t stack_waves = 0;
     size_t stack_cap = 0;
+
+    // block time steps (allocated on the first adaptive step; permuted with
+    // the store): acceleration of the last force evaluation, level, tick of
+    // the last force evaluation
+    float* aacc[3] = {nullptr, nullptr, nullptr};
+    uint32_t* alevel = nullptr;
+    uint32_t* atick = nullptr;
+    float* aacc_b[3] = {nullptr, nullptr, nullptr};
+    uint32_t* alevel_b = nullptr;
+    uint32_t* atick_b = nullptr;
 
     double* stage = nullptr;  // 3n doubles host<->device staging
     double* diag = nullptr;   // reduction scratch
@@ -254,5 +281,12 @@ void launch_energy(Device& d, double* out_dev);
 void launch_potential(Device& d, const uint32_t* slots_dev, uint32_t ns, double eps,
                       double min_d2, double* out_dev);
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+// block time steps
+void launch_adaptive_init(Device& d, const AdaptiveParams& a, uint32_t tick);
+void launch_adaptive_drift(Device& d, const AdaptiveParams& a);
+// count (optional, device): += number of particles corrected
+void launch_adaptive_correct(Device& d, const AdaptiveParams& a, uint32_t tick,
+                             unsigned long long* count);
+void launch_gather_u32(Device& d, const uint32_t* src_by_slot, uint32_t* out_by_caller);
 
 }  // namespace bh

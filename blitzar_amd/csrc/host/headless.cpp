@@ -211,7 +211,9 @@ int main(int argc, char** argv)
     p.min_theta = cfg.min_theta;
     p.max_acceleration = cfg.max_acceleration;
     p.max_speed = cfg.sph_max_speed;
-    p.criterion = cfg.criterion == "bounds" ? BH_HIP_CRITERION_BOUNDS : BH_HIP_CRITERION_COM;
+    p.criterion = cfg.criterion == "bounds"   ? BH_HIP_CRITERION_BOUNDS
+                  : cfg.criterion == "extent" ? BH_HIP_CRITERION_EXTENT
+                                              : BH_HIP_CRITERION_COM;
     p.leaf_capacity = static_cast<int32_t>(cfg.leaf_capacity);
     p.bucket_capacity = static_cast<int32_t>(cfg.bucket);
     p.multipole_order = cfg.multipole_order;

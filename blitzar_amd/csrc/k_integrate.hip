@@ -300,4 +300,157 @@ void launch_potential(Device& d, const uint32_t* slots_dev, uint32_t ns, double 
                                                d.n, eps * eps, min_d2, out_dev);
 }
 
+// ---- dyadic block time steps -------------------------------------------------
+// Semantics of engine/physics/core/cuda/integration/CudAdaptive.inl:31-57
+// (level rule), 147-172 (drift), 181-228 (activity, correction); FP64 state,
+// FP32 accelerations.
+namespace {
+
+__device__ inline uint32_t adaptive_level(float ax, float ay, float az, double vx, double vy,
+                                          double vz, const AdaptiveParams& a)
+{
+    const double am = sqrt(static_cast<double>(ax) * ax + static_cast<double>(ay) * ay +
+                           static_cast<double>(az) * az);
+    const double vm = sqrt(vx * vx + vy * vy + vz * vz);
+    const double adt = am > 1e-6 ? a.eta * sqrt(a.soft / am) : a.dt;
+    const double vdt = a.velocity && vm > 1e-6 ? a.eta * a.soft / vm : a.dt;
+    const double stable = fmin(a.dt, fmin(adt, vdt));
+    for (int l = 0; l <= a.levels; ++l)
+        if (a.dt / static_cast<double>(1u << l) <= stable)
+            return static_cast<uint32_t>(l);
+    return static_cast<uint32_t>(a.levels);
+}
+
+__global__ __launch_bounds__(256) void adaptive_init_kernel(
+    const float* __restrict__ ax, const float* __restrict__ ay, const float* __restrict__ az,
+    const double* __restrict__ vx, const double* __restrict__ vy, const double* __restrict__ vz,
+    float* __restrict__ bx, float* __restrict__ by, float* __restrict__ bz,
+    uint32_t* __restrict__ level, uint32_t* __restrict__ last, size_t n, AdaptiveParams a,
+    uint32_t tick)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    bx[i] = ax[i];
+    by[i] = ay[i];
+    bz[i] = az[i];
+    level[i] = adaptive_level(ax[i], ay[i], az[i], vx[i], vy[i], vz[i], a);
+    last[i] = tick;
+}
+
+// Predictor for every particle: x += v q + a q^2 / 2, v = clamp(v + a q).
+__global__ __launch_bounds__(256) void adaptive_drift_kernel(
+    double* __restrict__ px, double* __restrict__ py, double* __restrict__ pz,
+    double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
+    const float* __restrict__ ax, const float* __restrict__ ay, const float* __restrict__ az,
+    size_t n, double q, double vmax)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const double hq = 0.5 * q * q;
+    const double gx = ax[i], gy = ay[i], gz = az[i];
+    double x = vx[i], y = vy[i], z = vz[i];
+    px[i] += x * q + gx * hq;
+    py[i] += y * q + gy * hq;
+    pz[i] += z * q + gz * hq;
+    x += gx * q;
+    y += gy * q;
+    z += gz * q;
+    clamp3(x, y, z, vmax);
+    vx[i] = x;
+    vy[i] = y;
+    vz[i] = z;
+}
+
+// Corrector for the particles active at `tick` (their new force is in acc).
+__global__ __launch_bounds__(256) void adaptive_correct_kernel(
+    double* __restrict__ px, double* __restrict__ py, double* __restrict__ pz,
+    double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
+    const float* __restrict__ nx, const float* __restrict__ ny, const float* __restrict__ nz,
+    float* __restrict__ bx, float* __restrict__ by, float* __restrict__ bz,
+    uint32_t* __restrict__ level, uint32_t* __restrict__ last, size_t n, AdaptiveParams a,
+    uint32_t tick, unsigned long long* __restrict__ count)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    bool due = false;
+    if (i < n) {
+        const uint32_t lv = min(level[i], static_cast<uint32_t>(a.levels));
+        const uint32_t cadence = 1u << (static_cast<uint32_t>(a.levels) - lv);
+        due = (tick & (cadence - 1u)) == 0u;
+    }
+    if (count) {
+        const unsigned long long m = __ballot(due);
+        if ((threadIdx.x & 63) == 0 && m)
+            atomicAdd(count, static_cast<unsigned long long>(__popcll(m)));
+    }
+    if (!due)
+        return;
+    const double h = static_cast<double>(tick - last[i]) * a.quantum;
+    const double cx = static_cast<double>(nx[i]) - bx[i];
+    const double cy = static_cast<double>(ny[i]) - by[i];
+    const double cz = static_cast<double>(nz[i]) - bz[i];
+    const double hh = 0.5 * h * h;
+    px[i] += cx * hh;
+    py[i] += cy * hh;
+    pz[i] += cz * hh;
+    double x = vx[i] + cx * h, y = vy[i] + cy * h, z = vz[i] + cz * h;
+    clamp3(x, y, z, a.max_speed);
+    vx[i] = x;
+    vy[i] = y;
+    vz[i] = z;
+    bx[i] = nx[i];
+    by[i] = ny[i];
+    bz[i] = nz[i];
+    last[i] = tick;
+    level[i] = adaptive_level(nx[i], ny[i], nz[i], x, y, z, a);
+}
+
+__global__ __launch_bounds__(256) void gather_u32_kernel(const uint32_t* __restrict__ src,
+                                                         const uint32_t* __restrict__ rank,
+                                                         size_t n, uint32_t* __restrict__ out)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+        out[i] = src[rank[i]];
+}
+
+}  // namespace
+
+void launch_adaptive_init(Device& d, const AdaptiveParams& a, uint32_t tick)
+{
+    if (d.n == 0)
+        return;
+    adaptive_init_kernel<<<blocks_for(d.n), 256, 0, d.stream>>>(
+        d.acc[0], d.acc[1], d.acc[2], d.vel[0], d.vel[1], d.vel[2], d.aacc[0], d.aacc[1],
+        d.aacc[2], d.alevel, d.atick, d.n, a, tick);
+}
+
+void launch_adaptive_drift(Device& d, const AdaptiveParams& a)
+{
+    if (d.n == 0)
+        return;
+    adaptive_drift_kernel<<<blocks_for(d.n), 256, 0, d.stream>>>(
+        d.pos[0], d.pos[1], d.pos[2], d.vel[0], d.vel[1], d.vel[2], d.aacc[0], d.aacc[1],
+        d.aacc[2], d.n, a.quantum, a.max_speed);
+}
+
+void launch_adaptive_correct(Device& d, const AdaptiveParams& a, uint32_t tick,
+                             unsigned long long* count)
+{
+    if (d.n == 0)
+        return;
+    adaptive_correct_kernel<<<blocks_for(d.n), 256, 0, d.stream>>>(
+        d.pos[0], d.pos[1], d.pos[2], d.vel[0], d.vel[1], d.vel[2], d.acc[0], d.acc[1], d.acc[2],
+        d.aacc[0], d.aacc[1], d.aacc[2], d.alevel, d.atick, d.n, a, tick, count);
+}
+
+void launch_gather_u32(Device& d, const uint32_t* src_by_slot, uint32_t* out_by_caller)
+{
+    if (d.n == 0)
+        return;
+    gather_u32_kernel<<<blocks_for(d.n), 256, 0, d.stream>>>(src_by_slot, d.rank, d.n,
+                                                             out_by_caller);
+}
+
 }  // namespace bh

@@ -325,6 +325,24 @@ __global__ __launch_bounds__(256) void permute_kernel(
     oid[s] = id[j];
 }
 
+// The block time-step arrays follow the store through the same permutation.
+__global__ __launch_bounds__(256) void permute_adaptive_kernel(
+    const uint32_t* __restrict__ src, size_t n, const float* __restrict__ ax,
+    const float* __restrict__ ay, const float* __restrict__ az, const uint32_t* __restrict__ lv,
+    const uint32_t* __restrict__ lt, float* __restrict__ oax, float* __restrict__ oay,
+    float* __restrict__ oaz, uint32_t* __restrict__ olv, uint32_t* __restrict__ olt)
+{
+    const size_t s = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= n)
+        return;
+    const uint32_t j = src[s];
+    oax[s] = ax[j];
+    oay[s] = ay[j];
+    oaz[s] = az[j];
+    olv[s] = lv[j];
+    olt[s] = lt[j];
+}
+
 // rank[id[s]] = s: caller index -> store slot (rebuilt only when an API call
 // needs it, not every step).
 __global__ __launch_bounds__(256) void rank_kernel(const uint32_t* __restrict__ id, size_t n,
@@ -418,6 +436,15 @@ void launch_permute(Device& d)
     }
     std::swap(d.mass, d.mass_b);
     std::swap(d.id, d.id_b);
+    if (d.alevel) {
+        permute_adaptive_kernel<<<blocks, 256, 0, d.stream>>>(
+            d.vals, d.n, d.aacc[0], d.aacc[1], d.aacc[2], d.alevel, d.atick, d.aacc_b[0],
+            d.aacc_b[1], d.aacc_b[2], d.alevel_b, d.atick_b);
+        for (int a = 0; a < 3; ++a)
+            std::swap(d.aacc[a], d.aacc_b[a]);
+        std::swap(d.alevel, d.alevel_b);
+        std::swap(d.atick, d.atick_b);
+    }
 }
 
 void launch_rank(Device& d)

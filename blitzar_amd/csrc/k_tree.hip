@@ -104,6 +104,47 @@ __device__ inline void child_bounds(const unsigned long long* __restrict__ keys,
     }
 }
 
+// child_bounds of one node [a, a + c) by a whole wavefront: lanes 8o..8o+7
+// (o = 1..7) bracket boundary o with 8 probes per step, so each step shrinks
+// the bracket ninefold (about 8 dependent key loads at 8M particles instead of
+// 7 binary searches of 23 loads each). Every lane returns all 9 bounds. Used
+// for levels with few nodes, where one thread per node would leave the
+// search a long chain of dependent loads.
+constexpr int kWavePerNodeMax = 4096;
+
+__device__ inline void wave_child_bounds(const unsigned long long* __restrict__ keys, uint32_t a,
+                                         uint32_t c, int shift, uint32_t* bnd)
+{
+    const int lane = threadIdx.x & 63;
+    const int o = lane >> 3, j = lane & 7;
+    uint32_t lo = a, hi = a + c;  // boundary o lies in [lo, hi]
+    for (;;) {
+        const bool open = o > 0 && lo < hi;
+        if (!__any(open))
+            break;
+        const uint32_t len = hi - lo;
+        const uint32_t pj = lo + static_cast<uint32_t>((static_cast<uint64_t>(len) * (j + 1)) / 9u);
+        const bool pred = open && static_cast<int>((keys[pj] >> shift) & 7u) >= o;
+        const unsigned long long m = __ballot(pred);
+        if (open) {
+            const uint32_t gm = static_cast<uint32_t>(m >> (8 * o)) & 0xffu;
+            const int t = gm ? static_cast<int>(__ffs(gm)) - 1 : 8;
+            const uint32_t p_t =
+                lo + static_cast<uint32_t>((static_cast<uint64_t>(len) * (t + 1)) / 9u);
+            const uint32_t p_prev =
+                t > 0 ? lo + static_cast<uint32_t>((static_cast<uint64_t>(len) * t) / 9u) : 0u;
+            const uint32_t nlo = t > 0 ? p_prev + 1u : lo;
+            const uint32_t nhi = t < 8 ? p_t : hi;
+            lo = nlo;
+            hi = nhi;
+        }
+    }
+    bnd[0] = a;
+    bnd[8] = a + c;
+    for (int q = 1; q < 8; ++q)
+        bnd[q] = __shfl(lo, 8 * q, 64);  // lane 8q holds boundary q
+}
+
 __global__ __launch_bounds__(256) void tree_count_kernel(
     const unsigned long long* __restrict__ keys, const uint32_t* __restrict__ nstart,
     const uint32_t* __restrict__ ncount, uint32_t* __restrict__ nchild,
@@ -113,20 +154,28 @@ __global__ __launch_bounds__(256) void tree_count_kernel(
     const int base = level_base(info)[level];
     const int cnt = level_count(info)[level];
     const int shift = 3 * (kMaxLevel - level - 1);
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
+    const bool wave_mode = cnt <= kWavePerNodeMax;  // uniform across the grid
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int stride = wave_mode ? (gridDim.x * blockDim.x) >> 6 : gridDim.x * blockDim.x;
+    for (int k = wave_mode ? tid >> 6 : tid; k < cnt; k += stride) {
         const int node = base + k;
         const uint32_t a = nstart[node], c = ncount[node];
         const bool split = level < kMaxLevel && (level < leaf_depth || c > bucket);
         uint32_t nc = 0;
-        if (split) {
+        if (split) {  // uniform across the wave in wave mode
             uint32_t bnd[9];
-            child_bounds(keys, a, a + c, shift, bnd);
+            if (wave_mode)
+                wave_child_bounds(keys, a, c, shift, bnd);
+            else
+                child_bounds(keys, a, a + c, shift, bnd);
             for (int o = 0; o < 8; ++o)
                 nc += bnd[o + 1] > bnd[o] ? 1u : 0u;
         }
-        nchild[node] = nc;
-        nfirst[node] = a;  // leaves keep their particle start; parents are overwritten
-        scan_tmp[k] = nc;
+        if (!wave_mode || (threadIdx.x & 63) == 0) {
+            nchild[node] = nc;
+            nfirst[node] = a;  // leaves keep their particle start; parents are overwritten
+            scan_tmp[k] = nc;
+        }
     }
 }
 
@@ -164,20 +213,30 @@ __global__ __launch_bounds__(256) void tree_emit_kernel(
     const int next_base = info[level + 1];
     const bool dropped = info[kLevels + level + 1] == 0;
     const int shift = 3 * (kMaxLevel - level - 1);
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
+    const bool wave_mode = cnt <= kWavePerNodeMax;  // uniform across the grid
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int stride = wave_mode ? (gridDim.x * blockDim.x) >> 6 : gridDim.x * blockDim.x;
+    for (int k = wave_mode ? tid >> 6 : tid; k < cnt; k += stride) {
         const int node = base + k;
         const uint32_t nc = nchild[node];
         if (nc == 0)
-            continue;
+            continue;  // uniform across the wave in wave mode
+        const bool lead = !wave_mode || (threadIdx.x & 63) == 0;
         if (dropped) {
-            nchild[node] = 0;
+            if (lead)
+                nchild[node] = 0;
             continue;
         }
         const uint32_t first = static_cast<uint32_t>(next_base) + scan_tmp[k];
-        nfirst[node] = first;
         const uint32_t a = nstart[node], c = ncount[node];
         uint32_t bnd[9];
-        child_bounds(keys, a, a + c, shift, bnd);
+        if (wave_mode)
+            wave_child_bounds(keys, a, c, shift, bnd);
+        else
+            child_bounds(keys, a, a + c, shift, bnd);
+        if (!lead)
+            continue;
+        nfirst[node] = first;
         const unsigned long long p = nprefix[node];
         uint32_t j = 0;
         for (int o = 0; o < 8; ++o) {
@@ -426,7 +485,7 @@ void launch_tree_build(Device& d, int leaf_depth, int bucket)
     const int cap = static_cast<int>(d.node_cap);
     for (int L = 0; L < kMaxLevel; ++L) {
         const size_t bound = level_bound(L, d.n, d.node_cap);
-        const unsigned g = level_grid(bound);
+        const unsigned g = level_grid(bound * 64);  // a wavefront per node on small levels
         tree_count_kernel<<<g, 256, 0, d.stream>>>(keys, d.nstart, d.ncount, d.nchild, d.nfirst,
                                                    d.scan_tmp, d.level_info, L, leaf_depth,
                                                    static_cast<uint32_t>(bucket));

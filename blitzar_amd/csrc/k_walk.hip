@@ -6,6 +6,9 @@
 // at its centre of mass when the target is not inside it and
 //   size / sqrt(|com - x|^2 + eps^2) < theta      (criterion "com"), or
 //   size / dist(x, box)               < theta      (criterion "bounds");
+// octree_hip adds criterion "extent": bmax / sqrt(|com - x|^2 + eps^2) < theta,
+// bmax the radius about the COM that encloses the node's particles (the
+// multipole truncation error scales with bmax / distance, not cell size);
 // otherwise it is opened; particles of opened leaves are summed directly
 // (pair force engine/physics/octree/force/OctreeForce.cpp:23-57); the sum is
 // clamped to max_acceleration (engine/physics/octree/force/OctForces.cpp:128-133).
@@ -186,6 +189,17 @@ __global__ __launch_bounds__(64) void walk_kernel(
         const int gnode = static_cast<int>(gr.z);
         const int rcx = nodes[gnode].cx, rcy = nodes[gnode].cy, rcz = nodes[gnode].cz;
 
+        if (P.alevel) {
+            // block time steps: skip groups with no particle due a force now
+            bool due = false;
+            if (active) {
+                const uint32_t lv = min(P.alevel[s], static_cast<uint32_t>(P.alevels));
+                due = (P.atick & ((1u << (static_cast<uint32_t>(P.alevels) - lv)) - 1u)) == 0u;
+            }
+            if (!__any(due))
+                continue;
+        }
+
         float tx = 0.0f, ty = 0.0f, tz = 0.0f;
         if (active) {
             const float4 p = spart[s];
@@ -256,6 +270,8 @@ __global__ __launch_bounds__(64) void walk_kernel(
                         const float bz = fmaxf(fabsf(sz - gcz) - ghz - hs * wfz, 0.0f);
                         const float b2 = bx * bx + by * by + bz * bz;
                         ok = ok && b2 > 0.0f && size * size < P.theta2 * b2;
+                    } else if (CRIT == 2) {
+                        ok = ok && w.bmax * w.bmax < P.theta2 * (d2 + P.eps2);
                     } else {
                         ok = ok && size * size < P.theta2 * (d2 + P.eps2);
                     }
@@ -433,6 +449,8 @@ void launch_walk_record(Device& d, const WalkParams& p, unsigned long long* stat
     const unsigned grid = static_cast<unsigned>(d.stack_waves);
     if (p.criterion == 1)
         launch_walk_t<2, 1, false, true>(d, p, grid, stats, &L, band, min_count);
+    else if (p.criterion == 2)
+        launch_walk_t<2, 2, false, true>(d, p, grid, stats, &L, band, min_count);
     else
         launch_walk_t<2, 0, false, true>(d, p, grid, stats, &L, band, min_count);
 }
@@ -444,7 +462,14 @@ void launch_walk(Device& d, const WalkParams& p, unsigned long long* stats)
     launch_fill_u32(d.work_counter, 0u, 1, d.stream);
     const unsigned grid = static_cast<unsigned>(d.stack_waves);
     const bool tpm = p.rs > 0.0f;
-    if (p.order >= 2) {
+    if (p.criterion == 2) {
+        if (p.order >= 2)
+            tpm ? launch_walk_t<2, 2, true>(d, p, grid, stats)
+                : launch_walk_t<2, 2, false>(d, p, grid, stats);
+        else
+            tpm ? launch_walk_t<1, 2, true>(d, p, grid, stats)
+                : launch_walk_t<1, 2, false>(d, p, grid, stats);
+    } else if (p.order >= 2) {
         if (p.criterion == 1)
             tpm ? launch_walk_t<2, 1, true>(d, p, grid, stats)
                 : launch_walk_t<2, 1, false>(d, p, grid, stats);

@@ -17,7 +17,7 @@ from . import _abi
 from ._abi import Params, Timings, lib
 
 SOLVER_NAMES = ("octree_hip", "octree-hip", "octree_gpu_hip")
-CRITERIA = {"com": 0, "center_of_mass": 0, "bounds": 1}
+CRITERIA = {"com": 0, "center_of_mass": 0, "bounds": 1, "extent": 2}
 INTEGRATORS = {"leapfrog": 0, "kdk": 0, "euler": 1}
 
 
@@ -193,6 +193,12 @@ class OctreeHip:
         _check(lib().bh_hip_get_timings(self._h, C.byref(t)), self._h, "get_timings")
         return {k: getattr(t, k) for k, _ in Timings._fields_}
 
+
+    def adaptive_levels(self) -> np.ndarray:
+        """Block time-step level of every particle (caller order)."""
+        out = np.zeros(self.n, np.int32)
+        _check(lib().bh_hip_adaptive_levels(self._h, out), self._h, "adaptive_levels")
+        return out
 
     # -- learned far-field correction -----------------------------------------
     def lff_configure(self, theta: float = 0.9, band: float = 0.5, list_cap: int = 192,

@@ -41,7 +41,9 @@ typedef enum bh_hip_result {
 } bh_hip_result_t;
 
 enum { BH_HIP_INTEGRATOR_LEAPFROG = 0, BH_HIP_INTEGRATOR_EULER = 1 };
-enum { BH_HIP_CRITERION_COM = 0, BH_HIP_CRITERION_BOUNDS = 1 };
+/* COM and BOUNDS are the reference's octree(criterion) values; EXTENT opens
+ * by bmax / distance (bmax = radius about the COM enclosing the node). */
+enum { BH_HIP_CRITERION_COM = 0, BH_HIP_CRITERION_BOUNDS = 1, BH_HIP_CRITERION_EXTENT = 2 };
 enum { BH_HIP_KEYS_MORTON63 = 0, BH_HIP_KEYS_REFERENCE30 = 1 };
 
 /*
@@ -69,8 +71,17 @@ typedef struct bh_hip_params {
     float treepm_split_scale;/* r_s of the erfc short-range split; 0 = Newton */
     float treepm_cutoff;     /* short-range cutoff radius; 0 = none */
     int32_t waves_per_cu;    /* walk residency per CU (0 = default) */
-    int32_t reserved[7];
+    /* Dyadic block time steps (ParticleSystem::setAdaptiveTimeStepParameters,
+     * engine/physics/octree/adaptive/OctAdaptive.cpp:17-133): 0 = off, else
+     * each bh_hip_step of dt runs 2^min(max_level, 12) slices. */
+    int32_t adaptive_max_level;
+    float adaptive_eta;      /* level rule: eta sqrt(eps / |a|) (and eta eps / |v|) */
+    int32_t adaptive_flags;  /* BH_HIP_ADAPTIVE_* */
+    int32_t reserved[4];
 } bh_hip_params_t;
+
+/* adaptive_flags: also bound the step by eta * eps / |v| (the reference does) */
+enum { BH_HIP_ADAPTIVE_VELOCITY = 1 };
 
 typedef struct bh_hip_handle bh_hip_handle_t;
 
@@ -89,6 +100,7 @@ typedef struct bh_hip_timings {
     int32_t max_level;        /* deepest populated level of the last tree */
     int32_t walks;            /* force evaluations the sums above cover */
     int32_t groups;           /* walk groups (target sets) of the last tree */
+    uint64_t particle_steps;  /* block time steps: particles corrected (sum over slices) */
 } bh_hip_timings_t;
 
 /* Fill `out` with the defaults (reference config defaults, CfgConfig.hpp). */
@@ -211,6 +223,11 @@ bh_hip_result_t bh_hip_lff_forward(bh_hip_handle_t* h, const float* X, uint32_t 
  * With X and Y both NULL, *n receives the number of tuples held. */
 bh_hip_result_t bh_hip_lff_get_tuples(bh_hip_handle_t* h, float* X, float* Y, uint64_t cap,
                                       uint64_t* n);
+
+/* Per-particle block time-step level (caller order) of the adaptive
+ * schedule: the particle steps with dt / 2^level. Zero before the first
+ * adaptive step. */
+bh_hip_result_t bh_hip_adaptive_levels(bh_hip_handle_t* h, int32_t* levels);
 
 /* Standalone stable radix sort of (key, value) pairs on the device, the
  * building block behind thrust::sort_by_key at OctBuild.inl:96. Sorts by the

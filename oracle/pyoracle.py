@@ -99,6 +99,14 @@ def lib():
         L.dx_energy.argtypes = [C.c_int, _f64p, _f64p, _f64p, C.c_double, C.c_double, _f64p,
                                 C.c_int]
         L.dx_moments.argtypes = [C.c_int, _f64p, _f64p, _f64p]
+        _i32p_ = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        L.ad_steps.argtypes = [C.c_int, _f64p, _f64p, _f64p, _f64p, _i32p_, _u64p,
+                               C.POINTER(C.c_uint64), C.c_double, C.c_int, C.c_int, C.c_double,
+                               C.c_double, C.c_int, C.c_double, C.c_double, C.c_int]
+        L.ad_steps.restype = None
+        L.ad_choose_level.argtypes = [_f64p, _f64p, C.c_double, C.c_double, C.c_double, C.c_int,
+                                      C.c_int]
+        L.ad_choose_level.restype = C.c_int
         L.dx_multipole_accel.argtypes = [_f64p, _f64p, C.c_double, C.c_int, _f64p]
         _lib = L
     return _lib
@@ -342,3 +350,34 @@ def ref_octree_forces_subset(xyzm, targets, params=None, nthreads=0):
                                p.min_theta, p.criterion, t.size, t, acc.reshape(-1), nthreads,
                                C.byref(tb), C.byref(tw))
     return acc, tb.value, tw.value
+
+
+class AdaptiveState:
+    """Dyadic block time-step schedule of oracle/adaptive.cpp (FP64 state,
+    FP64 direct-sum forces)."""
+
+    def __init__(self, pos, vel, mass, dt, max_level, eta, soft, eps, velocity_criterion=True,
+                 max_speed=0.0):
+        self.pos = np.ascontiguousarray(pos, np.float64).copy()
+        self.vel = np.ascontiguousarray(vel, np.float64).copy()
+        self.mass = np.ascontiguousarray(mass, np.float64)
+        n = len(self.mass)
+        self.acc = np.zeros((n, 3), np.float64)
+        self.level = np.zeros(n, np.int32)
+        self.last = np.zeros(n, np.uint64)
+        self.tick = C.c_uint64(0)
+        self.args = (dt, max_level, eta, soft, 1 if velocity_criterion else 0, max_speed, eps)
+        self.primed = False
+
+    def steps(self, nsteps):
+        dt, L, eta, soft, vc, vmax, eps = self.args
+        lib().ad_steps(len(self.mass), self.pos.reshape(-1), self.vel.reshape(-1), self.mass,
+                       self.acc.reshape(-1), self.level, self.last, C.byref(self.tick), dt,
+                       nsteps, L, eta, soft, vc, vmax, eps, 0 if self.primed else 1)
+        self.primed = True
+
+
+def choose_level(a3, v3, eta, soft, dt, max_level, velocity_criterion=True):
+    return int(lib().ad_choose_level(np.ascontiguousarray(a3, np.float64),
+                                     np.ascontiguousarray(v3, np.float64), eta, soft, dt,
+                                     max_level, 1 if velocity_criterion else 0))
