@@ -86,6 +86,7 @@ def cpu_baseline(pos, m, theta, eps, n_targets, seed=123):
         "unit": "particle-steps/s",
         "cores": threads,
         "kind": kind,
+        "cpu_model": cpu_model(),
         "sample": (f"octree_cpu (theta {theta}, eps {eps}, monopole, FP32) on the same state: "
                    f"serial tree build over all {n} particles ({t_build:.2f} s) + walk on "
                    f"{n_targets} random targets ({t_walk:.2f} s, {threads} threads), "
@@ -94,9 +95,98 @@ def cpu_baseline(pos, m, theta, eps, n_targets, seed=123):
 
 
 def force_error(acc, ref, samples):
+    acc = np.asarray(acc, np.float64)
     r = np.linalg.norm(acc - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), 1e-30)
     return {"median": float(np.median(r)), "p99": float(np.percentile(r, 99)),
-            "max": float(r.max()), "samples": int(samples), "sample_seed": 2024}
+            "max": float(r.max()),
+            "l2": float(np.linalg.norm(acc - ref) / max(np.linalg.norm(ref), 1e-300)),
+            "samples": int(samples), "sample_seed": 2024}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def small_configs(with_cpu):
+    """The two smaller reference configurations, each on its own handle:
+    Plummer 65,536 (seed 42, eps 0.01, theta 0.5, KDK dt 1e-3, 100 steps, FP64
+    energy drift) and Plummer 1,048,576 (seed 7, eps 0.005, theta 0.5, dt 5e-4,
+    20 steps, forces at steps 0 and 20 against the FP64 direct sum and
+    octree_cpu)."""
+    import blitzar_amd as B
+    out = {}
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+    pos, vel, m = B.plummer(65536, seed=42)
+    with B.OctreeHip(65536, theta=0.5, softening=0.01, integrator="leapfrog") as s:
+        s.upload(pos, vel, m)
+        d0 = s.diagnostics(sample_limit=65536)
+        s.step(1e-3, 1)
+        s.upload(pos, vel, m)
+        t0 = time.perf_counter()
+        s.step(1e-3, 100)
+        el = time.perf_counter() - t0
+        d1 = s.diagnostics(sample_limit=65536)
+    e0, e1 = d0["ke"] + d0["pe"], d1["ke"] + d1["pe"]
+    c1 = {"workload": "plummer_65536_seed42_eps0.01_theta0.5_kdk_dt1e-3",
+          "steps": 100, "value": 65536 * 100 / el, "unit": "particle-steps/s",
+          "ms_per_step": el * 10.0, "energy_drift": abs(e1 - e0) / abs(e0)}
+    if with_cpu:
+        from oracle import pyoracle as O
+        x = O.hot(pos, m)
+        params = O.make_params(theta=0.5, softening=0.01)
+        ref = O.ref_octree_forces_subset(x, np.arange(65536), params, nthreads=threads)
+        if ref is not None:
+            _, tb, tw = ref
+            sub = np.arange(0, 65536, 16)
+            _, tb1, tw1 = O.ref_octree_forces_subset(x, sub, params, nthreads=1)
+            c1["cpu_baseline"] = {
+                "value": 65536 / (tb + tw), "unit": "particle-steps/s", "cores": threads,
+                "kind": "reference", "cpu_model": cpu_model(),
+                "single_thread_value": 65536 / (tb1 + tw1 * 16),
+                "sample": "octree_cpu full force pass (one per KDK step) over all 65,536; "
+                          "single thread: 4,096 targets extrapolated"}
+    out["plummer_65536"] = c1
+
+    n2 = 1 << 20
+    pos, vel, m = B.plummer(n2, seed=7)
+    rng = np.random.default_rng(2024)
+    t = np.sort(rng.choice(n2, 4096, replace=False)).astype(np.uint32)
+    with B.OctreeHip(n2, theta=0.5, softening=0.005, integrator="leapfrog") as s:
+        s.upload(pos, vel, m)
+        a0 = s.compute_accelerations()
+        r0 = s.direct_forces_fp64(t)
+        p0 = pos
+        t0 = time.perf_counter()
+        s.step(5e-4, 20)
+        el = time.perf_counter() - t0
+        a20 = s.compute_accelerations()
+        r20 = s.direct_forces_fp64(t)
+        p20, _ = s.download()
+    c2 = {"workload": "plummer_1048576_seed7_eps0.005_theta0.5_kdk_dt5e-4",
+          "steps": 20, "value": n2 * 20 / el, "unit": "particle-steps/s",
+          "ms_per_step": el * 1e3 / 20, "force_error_step0": force_error(a0[t], r0, 4096),
+          "force_error_step20": force_error(a20[t], r20, 4096)}
+    if with_cpu:
+        from oracle import pyoracle as O
+        params = O.make_params(theta=0.5, softening=0.005)
+        for tag, pp, aa in (("step0", p0, a0), ("step20", p20, a20)):
+            ref = O.ref_octree_forces_subset(O.hot(pp, m), t.astype(np.int32), params,
+                                             nthreads=threads)
+            if ref is not None:
+                rc = ref[0].astype(np.float64)
+                d = np.linalg.norm(aa[t] - rc, axis=1) / np.linalg.norm(rc, axis=1)
+                c2[f"vs_octree_cpu_{tag}"] = {"median": float(np.median(d)),
+                                             "p99": float(np.percentile(d, 99))}
+    out["plummer_1048576"] = c2
+    return out
 
 
 def block_time_steps(s, args, n):
@@ -179,6 +269,8 @@ def main(argv=None):
     ap.add_argument("--cpu-targets", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the 65,536 and 1,048,576 Plummer configurations")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -258,6 +350,12 @@ def main(argv=None):
         log(f"[bench] cpu baseline {cpu} ({time.perf_counter() - t0:.1f} s)")
     s.close()
 
+    configs = None
+    if not args.no_configs:
+        t0 = time.perf_counter()
+        configs = small_configs(not args.no_cpu)
+        log(f"[bench] smaller configurations ({time.perf_counter() - t0:.1f} s): {configs}")
+
     value = n * args.steps / elapsed
     line = {
         "metric": METRIC,
@@ -298,6 +396,8 @@ def main(argv=None):
         line["learned_far_field"] = lff_line
     if block_line is not None:
         line["block_time_steps"] = block_line
+    if configs is not None:
+        line["configs"] = configs
     print(json.dumps(line), flush=True)
     return 0
 

@@ -39,7 +39,9 @@ class Params(C.Structure):
         ("adaptive_max_level", C.c_int32),
         ("adaptive_eta", C.c_float),
         ("adaptive_flags", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("periodic_box", C.c_float),
+        ("pm_mesh", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
 
@@ -85,7 +87,7 @@ EXPORTS = [
     "bh_hip_get_timings", "bh_hip_get_last_error", "bh_hip_sort_pairs",
     "bh_hip_lff_configure", "bh_hip_lff_collect", "bh_hip_lff_train", "bh_hip_lff_get_params",
     "bh_hip_lff_set_params", "bh_hip_lff_gradient", "bh_hip_lff_forward",
-    "bh_hip_lff_get_tuples", "bh_hip_adaptive_levels",
+    "bh_hip_lff_get_tuples", "bh_hip_adaptive_levels", "bh_hip_pm_accelerations",
     "bh_ic_plummer", "bh_ic_hernquist", "bh_ic_hernquist_merger", "bh_ic_uniform_cube",
 ]
 
@@ -175,6 +177,8 @@ def lib() -> C.CDLL:
                                         C.POINTER(C.c_uint64)]
     L.bh_hip_adaptive_levels.argtypes = [H, _p(np.int32)]
     L.bh_hip_adaptive_levels.restype = R
+    L.bh_hip_pm_accelerations.argtypes = [H, _p(np.float32)]
+    L.bh_hip_pm_accelerations.restype = R
     for name in ("bh_hip_lff_configure", "bh_hip_lff_collect", "bh_hip_lff_train",
                  "bh_hip_lff_get_params", "bh_hip_lff_set_params", "bh_hip_lff_gradient",
                  "bh_hip_lff_forward", "bh_hip_lff_get_tuples"):

@@ -35,6 +35,8 @@ struct bh_hip_handle {
     int leaf_depth = 1;
     int cu_count = 256;
     bool primed = false;      // acc holds a(x) of the current positions
+    bh::PmState pm;           // particle mesh of the periodic split (pm_mesh > 0)
+    bool pm_failed = false;   // a rocFFT execution failed in the last call
     bool ad_primed = false;   // block time-step schedule initialised
     double ad_quantum = 0.0;  // slice length of the schedule
     uint64_t ad_tick = 0;     // slices since the schedule started
@@ -116,9 +118,29 @@ bh_hip_params_t resolve(const bh_hip_params_t& in)
     p.adaptive_max_level = std::clamp(in.adaptive_max_level, 0, 12);
     p.adaptive_eta = std::clamp(in.adaptive_eta, 0.01f, 1.0f);
     p.adaptive_flags = in.adaptive_flags & BH_HIP_ADAPTIVE_VELOCITY;
+    p.periodic_box = std::max(0.0f, in.periodic_box);
+    p.pm_mesh = std::max(0, in.pm_mesh);
     if (p.waves_per_cu <= 0)
         p.waves_per_cu = kDefaultWavesPerCu;
     return p;
+}
+
+// Parameter combinations the solver does not run; nullptr when valid.
+const char* invalid_combination(const bh_hip_params_t& p)
+{
+    if (p.periodic_box > 0.0f) {
+        if (!(p.treepm_split_scale > 0.0f) || !(p.treepm_cutoff > 0.0f))
+            return "periodic_box needs the TreePM short-range walk (split scale and cutoff > 0)";
+        if (p.treepm_cutoff * 4.0f > p.periodic_box)
+            return "periodic_box needs treepm_cutoff <= periodic_box / 4 (nearest image)";
+    }
+    if (p.pm_mesh > 0) {
+        if (!(p.periodic_box > 0.0f))
+            return "pm_mesh needs periodic_box > 0 and the short-range split";
+        if (p.pm_mesh < 8 || p.pm_mesh > 1024 || (p.pm_mesh & 1))
+            return "pm_mesh must be even and within [8, 1024]";
+    }
+    return nullptr;
 }
 
 // OctBuild.inl:28-39
@@ -211,6 +233,11 @@ bh_hip_result_t finish(bh_hip_handle* h, const char* what)
 {
     if (!check(h, hipGetLastError(), what) || !check(h, hipStreamSynchronize(h->d.stream), what))
         return BH_HIP_INTERNAL_ERROR;
+    if (h->pm_failed) {
+        h->pm_failed = false;
+        h->last_error = std::string("[octree_hip] ") + what + ": rocFFT execution failed";
+        return BH_HIP_INTERNAL_ERROR;
+    }
     int flags = 0;
     if (!check(h, hipMemcpy(&flags, h->d.err, sizeof(int), hipMemcpyDeviceToHost), "error read"))
         return BH_HIP_INTERNAL_ERROR;
@@ -294,6 +321,7 @@ bh::WalkParams walk_params(const bh_hip_params_t& p)
     w.order = p.multipole_order;
     w.rs = p.treepm_split_scale;
     w.rcut2 = p.treepm_cutoff > 0.0f ? p.treepm_cutoff * p.treepm_cutoff : 0.0f;
+    w.box = p.periodic_box;
     return w;
 }
 
@@ -325,6 +353,13 @@ void queue_force(bh_hip_handle* h)
         bh::launch_walk_record(h->d, wp, h->stats, h->lff, h->lff_cfg.band, h->lff_cfg.min_count);
         bh::launch_lff_apply(h->d, h->lff, h->lff_cfg.band, wp.eps2, h->lff_cfg.min_count,
                              h->p.max_acceleration);
+    } else if (h->pm.N > 0) {
+        bh::WalkParams wp = walk_params(h->p);
+        wp.max_acc = 0.0f;  // clamped after the mesh force
+        bh::launch_walk(h->d, wp, h->stats);
+        if (!bh::launch_pm_accel(h->d, h->pm, h->p.periodic_box, h->p.treepm_split_scale,
+                                 h->p.max_acceleration))
+            h->pm_failed = true;
     } else {
         bh::launch_walk(h->d, walk_params(h->p), h->stats);
     }
@@ -393,6 +428,7 @@ bh::AdaptiveParams adaptive_params(const bh_hip_handle* h, double dt)
     a.quantum = dt / static_cast<double>(1u << a.levels);
     a.velocity = (h->p.adaptive_flags & BH_HIP_ADAPTIVE_VELOCITY) != 0;
     a.max_speed = h->p.max_speed;
+    a.box = h->p.periodic_box;
     return a;
 }
 
@@ -513,6 +549,10 @@ bh_hip_result_t bh_hip_create(const bh_hip_params_t* params, uint64_t n, bh_hip_
         set_err(err, err_cap, "bh_hip_create: invalid argument (params/out null or n out of range)");
         return BH_HIP_INVALID_ARGUMENT;
     }
+    if (const char* why = invalid_combination(resolve(*params))) {
+        set_err(err, err_cap, (std::string("bh_hip_create: ") + why).c_str());
+        return BH_HIP_INVALID_ARGUMENT;
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
         set_err(err, err_cap, "bh_hip_create: no HIP device available");
@@ -533,6 +573,10 @@ bh_hip_result_t bh_hip_create(const bh_hip_params_t* params, uint64_t n, bh_hip_
     bool ok = check(h, hipStreamCreateWithFlags(&h->d.stream, hipStreamNonBlocking), "stream");
     ok = ok && alloc_all(h, n);
     ok = ok && check(h, hipEventCreate(&h->i0), "event") && check(h, hipEventCreate(&h->i1), "event");
+    if (ok && h->p.pm_mesh > 0 && !bh::pm_setup(h->pm, h->p.pm_mesh, h->d.stream)) {
+        h->last_error = "bh_hip_create: particle-mesh setup (rocFFT plans, meshes) failed";
+        ok = false;
+    }
     if (ok) {
         bh::launch_init_ids(h->d);
         ok = zero_err(h) && finish(h, "create") == BH_HIP_OK;
@@ -552,6 +596,7 @@ void bh_hip_destroy(bh_hip_handle_t* h)
         return;
     if (h->d.stream)
         (void)hipStreamSynchronize(h->d.stream);
+    bh::pm_release(h->pm);
     for (void* p : h->allocations)
         (void)hipFree(p);
     h->allocations.clear();
@@ -575,6 +620,10 @@ bh_hip_result_t bh_hip_set_params(bh_hip_handle_t* h, const bh_hip_params_t* par
         return BH_HIP_INVALID_ARGUMENT;
     const bh_hip_params_t old = h->p;
     const bh_hip_params_t next = resolve(*params);
+    if (const char* why = invalid_combination(next)) {
+        h->last_error = std::string("set_params: ") + why;
+        return BH_HIP_INVALID_ARGUMENT;
+    }
     if (h->lff_cfg.enabled) {
         if (const char* why = lff_incompatible(next)) {
             h->last_error = std::string("set_params: ") + why;
@@ -584,6 +633,18 @@ bh_hip_result_t bh_hip_set_params(bh_hip_handle_t* h, const bh_hip_params_t* par
     if (next.adaptive_max_level != old.adaptive_max_level ||
         next.adaptive_eta != old.adaptive_eta || next.adaptive_flags != old.adaptive_flags)
         h->ad_primed = false;
+    if (next.pm_mesh != h->pm.N) {
+        if (next.pm_mesh > 0) {
+            if (!bh::pm_setup(h->pm, next.pm_mesh, h->d.stream)) {
+                bh::pm_release(h->pm);
+                h->last_error = "set_params: particle-mesh setup (rocFFT plans, meshes) failed";
+                return BH_HIP_INTERNAL_ERROR;
+            }
+        } else {
+            (void)hipStreamSynchronize(h->d.stream);
+            bh::pm_release(h->pm);
+        }
+    }
     h->p = next;
     h->p.waves_per_cu = old.waves_per_cu;  // residency is fixed at creation
     h->leaf_depth = reference_leaf_depth(h->d.n, h->p.leaf_capacity);
@@ -604,6 +665,7 @@ bh_hip_result_t bh_hip_upload(bh_hip_handle_t* h, const double* pos3, const doub
     bool ok = zero_err(h) && stage_to_device(h, pos3);
     if (ok) {
         bh::launch_upload_scatter(d, 0);
+        bh::launch_wrap(d, h->p.periodic_box);
         ok = check(h, hipStreamSynchronize(d.stream), "upload pos");
     }
     if (ok) {
@@ -723,14 +785,14 @@ bh_hip_result_t bh_hip_step(bh_hip_handle_t* h, double dt, int32_t nsteps)
         if (h->p.integrator == BH_HIP_INTEGRATOR_EULER) {
             queue_force(h);
             rec(h->timing ? h->i0 : nullptr, d.stream);
-            bh::launch_kick_drift(d, dt, dt, vmax);
+            bh::launch_kick_drift(d, dt, dt, vmax, h->p.periodic_box);
             rec(h->timing ? h->i1 : nullptr, d.stream);
             h->primed = false;
         } else {
             if (!h->primed)
                 queue_force(h);
             rec(h->timing ? h->i0 : nullptr, d.stream);
-            bh::launch_kick_drift(d, 0.5 * dt, dt, vmax);
+            bh::launch_kick_drift(d, 0.5 * dt, dt, vmax, h->p.periodic_box);
             rec(h->timing ? h->i1 : nullptr, d.stream);
             if (h->timing) {
                 // integration time of this step: the kick-drift kernel (the
@@ -1338,5 +1400,37 @@ extern "C" bh_hip_result_t bh_hip_adaptive_levels(bh_hip_handle_t* h, int32_t* l
         !check(h, hipMemcpy(levels, tmp, d.n * sizeof(uint32_t), hipMemcpyDeviceToHost),
                "adaptive_levels"))
         return BH_HIP_INTERNAL_ERROR;
+    return BH_HIP_OK;
+}
+
+extern "C" bh_hip_result_t bh_hip_pm_accelerations(bh_hip_handle_t* h, float* acc3)
+{
+    if (!h || !acc3)
+        return BH_HIP_INVALID_ARGUMENT;
+    if (h->pm.N == 0) {
+        h->last_error = "pm_accelerations: no particle mesh (pm_mesh = 0)";
+        return BH_HIP_NOT_READY;
+    }
+    bh::Device& d = h->d;
+    if (!zero_err(h))
+        return BH_HIP_INTERNAL_ERROR;
+    for (int a = 0; a < 3; ++a)
+        bh::launch_fill_u32(reinterpret_cast<uint32_t*>(d.acc[a]), 0u, d.n, d.stream);
+    if (!bh::launch_pm_accel(d, h->pm, h->p.periodic_box, h->p.treepm_split_scale, 0.0f))
+        h->pm_failed = true;
+    h->primed = false;  // acc now holds the mesh part only
+    bh_hip_result_t r = finish(h, "pm_accelerations");
+    if (r != BH_HIP_OK)
+        return r;
+    ensure_rank(h);
+    bh::launch_download_gather(d, 2);
+    std::vector<double> tmp(3 * d.n);
+    if (!check(h, hipMemcpyAsync(tmp.data(), d.stage, tmp.size() * sizeof(double),
+                                 hipMemcpyDeviceToHost, d.stream),
+               "pm download") ||
+        !check(h, hipStreamSynchronize(d.stream), "pm download"))
+        return BH_HIP_INTERNAL_ERROR;
+    for (size_t i = 0; i < tmp.size(); ++i)
+        acc3[i] = static_cast<float>(tmp[i]);
     return BH_HIP_OK;
 }

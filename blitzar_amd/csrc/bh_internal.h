@@ -3,6 +3,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocfft/rocfft.h>
 #include <cstddef>
 #include <cstdint>
 
@@ -108,6 +109,7 @@ struct WalkParams {
     int order;          // 1 = monopole, 2 = quadrupole
     float rs;           // TreePM split scale (0 = Newton)
     float rcut2;        // TreePM cutoff squared (0 = none)
+    float box = 0.0f;   // periodic cube side for the TreePM walk (0 = open)
     // Block time steps: when alevel is set, only groups holding a particle
     // active at tick `atick` are walked (active: atick mod 2^(alevels -
     // level) == 0); the other groups keep their previous acc.
@@ -125,6 +127,7 @@ struct AdaptiveParams {
     double quantum;   // dt / 2^L
     int velocity;     // also bound by eta * soft / |v|
     double max_speed;
+    double box;       // periodic cube side (0 = open): positions wrapped after updates
 };
 
 // Device buffers of one solver instance. The particle store is kept in the
@@ -219,6 +222,19 @@ struct LffState {
     uint32_t collects = 0;                  // collect calls so far (hash seed)
 };
 
+// Particle-mesh state of one solver instance (k_pm.hip).
+struct PmState {
+    int N = 0;
+    rocfft_plan fwd = nullptr, inv = nullptr;
+    rocfft_execution_info info = nullptr;
+    void* work = nullptr;
+    size_t work_bytes = 0;
+    float* rho = nullptr;                    // N^3 density
+    float* rho_k = nullptr;                  // N^2 (N/2+1) complex
+    float* tmp_k = nullptr;                  // N^2 (N/2+1) complex
+    float* field[3] = {nullptr, nullptr, nullptr};  // N^3 acceleration components (x N^3)
+};
+
 // ---- launchers (defined in the kernel translation units) -------------------
 // k_keys.hip
 void launch_aabb(Device& d);
@@ -271,7 +287,10 @@ void launch_lff_apply(Device& d, const LffState& L, float band, float eps2, int 
                       float max_acc);
 // k_integrate.hip
 void launch_kick(Device& d, double dt_kick, double max_speed);
-void launch_kick_drift(Device& d, double dt_kick, double dt_drift, double max_speed);
+// box > 0: positions wrapped into [0, box) after the drift
+void launch_kick_drift(Device& d, double dt_kick, double dt_drift, double max_speed,
+                       double box = 0.0);
+void launch_wrap(Device& d, double box);
 void launch_upload_scatter(Device& d, int which);   // 0 = pos, 1 = vel
 void launch_download_gather(Device& d, int which);  // 0 = pos, 1 = vel, 2 = acc(float->double)
 void launch_init_ids(Device& d);
@@ -281,6 +300,11 @@ void launch_energy(Device& d, double* out_dev);
 void launch_potential(Device& d, const uint32_t* slots_dev, uint32_t ns, double eps,
                       double min_d2, double* out_dev);
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+// k_pm.hip: plans and meshes for an N^3 mesh (host allocation); the force
+// pass adds the long-range acceleration to acc and applies the clamp.
+bool pm_setup(PmState& pm, int N, hipStream_t stream);
+void pm_release(PmState& pm);
+bool launch_pm_accel(Device& d, PmState& pm, double L, float rs, float max_acc);
 // block time steps
 void launch_adaptive_init(Device& d, const AdaptiveParams& a, uint32_t tick);
 void launch_adaptive_drift(Device& d, const AdaptiveParams& a);

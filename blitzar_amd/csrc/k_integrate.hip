@@ -19,6 +19,15 @@
 namespace bh {
 namespace {
 
+// Position wrapped into [0, box) (box <= 0: unchanged).
+__device__ inline double wrap_box(double x, double box)
+{
+    if (box <= 0.0)
+        return x;
+    x -= box * floor(x / box);
+    return x < box ? x : 0.0;  // x / box rounding just below 1
+}
+
 __device__ inline void clamp3(double& x, double& y, double& z, double limit)
 {
     const double s = sqrt(x * x + y * y + z * z);
@@ -34,7 +43,7 @@ __global__ __launch_bounds__(256) void kick_drift_kernel(
     double* __restrict__ px, double* __restrict__ py, double* __restrict__ pz,
     double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
     const float* __restrict__ ax, const float* __restrict__ ay, const float* __restrict__ az,
-    size_t n, double dtk, double dtd, double vmax)
+    size_t n, double dtk, double dtd, double vmax, double box)
 {
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n)
@@ -47,10 +56,22 @@ __global__ __launch_bounds__(256) void kick_drift_kernel(
     vy[i] = y;
     vz[i] = z;
     if (dtd != 0.0) {
-        px[i] += x * dtd;
-        py[i] += y * dtd;
-        pz[i] += z * dtd;
+        px[i] = wrap_box(px[i] + x * dtd, box);
+        py[i] = wrap_box(py[i] + y * dtd, box);
+        pz[i] = wrap_box(pz[i] + z * dtd, box);
     }
+}
+
+__global__ __launch_bounds__(256) void wrap_kernel(double* __restrict__ px,
+                                                   double* __restrict__ py,
+                                                   double* __restrict__ pz, size_t n, double box)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    px[i] = wrap_box(px[i], box);
+    py[i] = wrap_box(py[i], box);
+    pz[i] = wrap_box(pz[i], box);
 }
 
 __global__ __launch_bounds__(256) void init_ids_kernel(uint32_t* __restrict__ id,
@@ -231,16 +252,23 @@ void launch_kick(Device& d, double dt_kick, double max_speed)
         return;
     kick_drift_kernel<<<blocks_for(d.n), 256, 0, d.stream>>>(
         d.pos[0], d.pos[1], d.pos[2], d.vel[0], d.vel[1], d.vel[2], d.acc[0], d.acc[1], d.acc[2],
-        d.n, dt_kick, 0.0, max_speed);
+        d.n, dt_kick, 0.0, max_speed, 0.0);
 }
 
-void launch_kick_drift(Device& d, double dt_kick, double dt_drift, double max_speed)
+void launch_kick_drift(Device& d, double dt_kick, double dt_drift, double max_speed, double box)
 {
     if (d.n == 0)
         return;
     kick_drift_kernel<<<blocks_for(d.n), 256, 0, d.stream>>>(
         d.pos[0], d.pos[1], d.pos[2], d.vel[0], d.vel[1], d.vel[2], d.acc[0], d.acc[1], d.acc[2],
-        d.n, dt_kick, dt_drift, max_speed);
+        d.n, dt_kick, dt_drift, max_speed, box);
+}
+
+void launch_wrap(Device& d, double box)
+{
+    if (d.n == 0 || box <= 0.0)
+        return;
+    wrap_kernel<<<blocks_for(d.n), 256, 0, d.stream>>>(d.pos[0], d.pos[1], d.pos[2], d.n, box);
 }
 
 void launch_init_ids(Device& d)
@@ -343,7 +371,7 @@ __global__ __launch_bounds__(256) void adaptive_drift_kernel(
     double* __restrict__ px, double* __restrict__ py, double* __restrict__ pz,
     double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
     const float* __restrict__ ax, const float* __restrict__ ay, const float* __restrict__ az,
-    size_t n, double q, double vmax)
+    size_t n, double q, double vmax, double box)
 {
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n)
@@ -351,9 +379,9 @@ __global__ __launch_bounds__(256) void adaptive_drift_kernel(
     const double hq = 0.5 * q * q;
     const double gx = ax[i], gy = ay[i], gz = az[i];
     double x = vx[i], y = vy[i], z = vz[i];
-    px[i] += x * q + gx * hq;
-    py[i] += y * q + gy * hq;
-    pz[i] += z * q + gz * hq;
+    px[i] = wrap_box(px[i] + x * q + gx * hq, box);
+    py[i] = wrap_box(py[i] + y * q + gy * hq, box);
+    pz[i] = wrap_box(pz[i] + z * q + gz * hq, box);
     x += gx * q;
     y += gy * q;
     z += gz * q;
@@ -391,9 +419,9 @@ __global__ __launch_bounds__(256) void adaptive_correct_kernel(
     const double cy = static_cast<double>(ny[i]) - by[i];
     const double cz = static_cast<double>(nz[i]) - bz[i];
     const double hh = 0.5 * h * h;
-    px[i] += cx * hh;
-    py[i] += cy * hh;
-    pz[i] += cz * hh;
+    px[i] = wrap_box(px[i] + cx * hh, a.box);
+    py[i] = wrap_box(py[i] + cy * hh, a.box);
+    pz[i] = wrap_box(pz[i] + cz * hh, a.box);
     double x = vx[i] + cx * h, y = vy[i] + cy * h, z = vz[i] + cz * h;
     clamp3(x, y, z, a.max_speed);
     vx[i] = x;
@@ -432,7 +460,7 @@ void launch_adaptive_drift(Device& d, const AdaptiveParams& a)
         return;
     adaptive_drift_kernel<<<blocks_for(d.n), 256, 0, d.stream>>>(
         d.pos[0], d.pos[1], d.pos[2], d.vel[0], d.vel[1], d.vel[2], d.aacc[0], d.aacc[1],
-        d.aacc[2], d.n, a.quantum, a.max_speed);
+        d.aacc[2], d.n, a.quantum, a.max_speed, a.box);
 }
 
 void launch_adaptive_correct(Device& d, const AdaptiveParams& a, uint32_t tick,

@@ -85,10 +85,17 @@ __device__ inline void eval_cells(const float4* __restrict__ ca, const float4* _
                                   float tz, const WalkParams& P, float& ax, float& ay, float& az)
 {
     const float inv_rs = TPM ? 1.0f / P.rs : 0.0f;
+    const float box = TPM ? P.box : 0.0f;
+    const float ib = box > 0.0f ? 1.0f / box : 0.0f;
 #pragma unroll 2
     for (int j = 0; j < ncell; ++j) {
         const float4 a = ca[j];
-        const float dx = a.x - tx, dy = a.y - ty, dz = a.z - tz;
+        float dx = a.x - tx, dy = a.y - ty, dz = a.z - tz;
+        if (TPM && box > 0.0f) {  // nearest image of this pair
+            dx -= box * rintf(dx * ib);
+            dy -= box * rintf(dy * ib);
+            dz -= box * rintf(dz * ib);
+        }
         const float r2 = dx * dx + dy * dy + dz * dz + P.eps2;
         const float ri = __frsqrt_rn(r2);
         const float ri2 = ri * ri;
@@ -124,10 +131,17 @@ __device__ inline void eval_parts(const float4* __restrict__ pl, int np, float t
                                   float tz, const WalkParams& P, float& ax, float& ay, float& az)
 {
     const float inv_rs = TPM ? 1.0f / P.rs : 0.0f;
+    const float box = TPM ? P.box : 0.0f;
+    const float ib = box > 0.0f ? 1.0f / box : 0.0f;
 #pragma unroll 4
     for (int j = 0; j < np; ++j) {
         const float4 a = pl[j];
-        const float dx = a.x - tx, dy = a.y - ty, dz = a.z - tz;
+        float dx = a.x - tx, dy = a.y - ty, dz = a.z - tz;
+        if (TPM && box > 0.0f) {  // nearest image of this pair
+            dx -= box * rintf(dx * ib);
+            dy -= box * rintf(dy * ib);
+            dz -= box * rintf(dz * ib);
+        }
         const float d2 = dx * dx + dy * dy + dz * dz;
         const float r2 = d2 + P.eps2;
         const float ri = __frsqrt_rn(r2);
@@ -247,6 +261,7 @@ __global__ __launch_bounds__(64) void walk_kernel(
             float px = 0.0f, py = 0.0f, pz = 0.0f;
             float size = 0.0f;
             bool near = false;
+            bool wide = false;
             if (node >= 0) {
                 w = nodes[node];
                 if (w.mass > 0.0f) {
@@ -256,6 +271,23 @@ __global__ __launch_bounds__(64) void walk_kernel(
                     px = sx + w.comx;
                     py = sy + w.comy;
                     pz = sz + w.comz;
+                    if (TPM && P.box > 0.0f) {
+                        if (4.0f * w.bmax > P.box) {
+                            wide = true;  // spans several images: always opened
+                        } else {
+                            // the node's image nearest to the group
+                            const float ib = 1.0f / P.box;
+                            const float hx = -P.box * rintf((px - gcx) * ib);
+                            const float hy = -P.box * rintf((py - gcy) * ib);
+                            const float hz = -P.box * rintf((pz - gcz) * ib);
+                            sx += hx;
+                            sy += hy;
+                            sz += hz;
+                            px += hx;
+                            py += hy;
+                            pz += hz;
+                        }
+                    }
                     const float dx = fmaxf(fabsf(px - gcx) - ghx, 0.0f);
                     const float dy = fmaxf(fabsf(py - gcy) - ghy, 0.0f);
                     const float dz = fmaxf(fabsf(pz - gcz) - ghz, 0.0f);
@@ -275,7 +307,7 @@ __global__ __launch_bounds__(64) void walk_kernel(
                     } else {
                         ok = ok && size * size < P.theta2 * (d2 + P.eps2);
                     }
-                    if (TPM && P.rcut2 > 0.0f) {
+                    if (TPM && P.rcut2 > 0.0f && !wide) {
                         // Prune nodes whose bounding sphere lies beyond the cutoff for the
                         // whole group.
                         const float lim = sqrtf(P.rcut2) + w.bmax;
@@ -284,6 +316,8 @@ __global__ __launch_bounds__(64) void walk_kernel(
                             w.mass = 0.0f;
                         }
                     }
+                    if (wide)
+                        ok = false;
                     if (REC && ok) {
                         // near the group: small nodes are opened, large ones
                         // accepted and recorded for the learned correction

@@ -194,6 +194,13 @@ class OctreeHip:
         return {k: getattr(t, k) for k, _ in Timings._fields_}
 
 
+    def pm_accelerations(self) -> np.ndarray:
+        """Long-range (mesh) part of the periodic TreePM force (caller order)."""
+        out = np.zeros((self.n, 3), np.float32)
+        _check(lib().bh_hip_pm_accelerations(self._h, out.reshape(-1)), self._h,
+               "pm_accelerations")
+        return out
+
     def adaptive_levels(self) -> np.ndarray:
         """Block time-step level of every particle (caller order)."""
         out = np.zeros(self.n, np.int32)

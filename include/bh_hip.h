@@ -77,7 +77,17 @@ typedef struct bh_hip_params {
     int32_t adaptive_max_level;
     float adaptive_eta;      /* level rule: eta sqrt(eps / |a|) (and eta eps / |v|) */
     int32_t adaptive_flags;  /* BH_HIP_ADAPTIVE_* */
-    int32_t reserved[4];
+    /* Periodic cube [0, L)^3 for the TreePM short-range walk (0 = open
+     * boundaries). Needs treepm_split_scale > 0 and 0 < treepm_cutoff <= L/4;
+     * every pair is taken at its nearest image and positions are wrapped
+     * into the box after each drift. The reference has no periodic
+     * short-range walk (its periodic mode is PM only). */
+    float periodic_box;
+    /* Particle-mesh long-range force of the periodic split (rocFFT): mesh
+     * cells per side (0 = short range only). Needs periodic_box and the
+     * short-range split; even, 8..1024. */
+    int32_t pm_mesh;
+    int32_t reserved[2];
 } bh_hip_params_t;
 
 /* adaptive_flags: also bound the step by eta * eps / |v| (the reference does) */
@@ -223,6 +233,12 @@ bh_hip_result_t bh_hip_lff_forward(bh_hip_handle_t* h, const float* X, uint32_t 
  * With X and Y both NULL, *n receives the number of tuples held. */
 bh_hip_result_t bh_hip_lff_get_tuples(bh_hip_handle_t* h, float* X, float* Y, uint64_t cap,
                                       uint64_t* n);
+
+/* Long-range (mesh) part of the periodic TreePM force at the current
+ * positions, caller order, n x 3 (needs pm_mesh > 0). No counterpart with the
+ * split in the reference (its periodic PM runs with r_s = 0,
+ * engine/physics/treepm/cuda/cosmology/TpmCosmology.inl:52). */
+bh_hip_result_t bh_hip_pm_accelerations(bh_hip_handle_t* h, float* acc3);
 
 /* Per-particle block time-step level (caller order) of the adaptive
  * schedule: the particle steps with dt / 2^level. Zero before the first

@@ -88,8 +88,22 @@ void dx_force_error(int nt, const double* approx3, const double* ref3, double* o
 // softened distance (engine/physics/octree/force/OctreeForce.cpp:49-57), and
 // pairs beyond the cutoff rc (unsoftened distance) dropped
 // (OctreeTraversal.cpp:172-176).
+// box > 0: periodic cube [0, box)^3, each pair at its nearest image
+// (separation wrapped into [-box/2, box/2] per axis). The reference has no
+// periodic short-range walk; this is the definition the GPU walk is held to.
+void dx_direct_tpm_fp64_box(int n, const double* pos3, const double* mass, int nt,
+                            const int* targets, double eps, double rs, double rc, double box,
+                            double* acc3, int nthreads);
+
 void dx_direct_tpm_fp64(int n, const double* pos3, const double* mass, int nt, const int* targets,
                         double eps, double rs, double rc, double* acc3, int nthreads)
+{
+    dx_direct_tpm_fp64_box(n, pos3, mass, nt, targets, eps, rs, rc, 0.0, acc3, nthreads);
+}
+
+void dx_direct_tpm_fp64_box(int n, const double* pos3, const double* mass, int nt,
+                            const int* targets, double eps, double rs, double rc, double box,
+                            double* acc3, int nthreads)
 {
     const int threads = nthreads > 0 ? nthreads : omp_get_max_threads();
     const double e2 = eps * eps;
@@ -102,8 +116,13 @@ void dx_direct_tpm_fp64(int n, const double* pos3, const double* mass, int nt, c
         for (int j = 0; j < n; ++j) {
             if (j == i)
                 continue;
-            const double dx = pos3[3 * j] - pos3[3 * i], dy = pos3[3 * j + 1] - pos3[3 * i + 1],
-                         dz = pos3[3 * j + 2] - pos3[3 * i + 2];
+            double dx = pos3[3 * j] - pos3[3 * i], dy = pos3[3 * j + 1] - pos3[3 * i + 1],
+                   dz = pos3[3 * j + 2] - pos3[3 * i + 2];
+            if (box > 0.0) {
+                dx -= box * std::nearbyint(dx / box);
+                dy -= box * std::nearbyint(dy / box);
+                dz -= box * std::nearbyint(dz / box);
+            }
             const double d2 = dx * dx + dy * dy + dz * dz;
             if (rc > 0.0 && d2 > rc2)
                 continue;

@@ -96,6 +96,8 @@ def lib():
         L.dx_force_error.argtypes = [C.c_int, _f64p, _f64p, _f64p]
         L.dx_direct_tpm_fp64.argtypes = [C.c_int, _f64p, _f64p, C.c_int, _i32p, C.c_double,
                                          C.c_double, C.c_double, _f64p, C.c_int]
+        L.dx_direct_tpm_fp64_box.argtypes = [C.c_int, _f64p, _f64p, C.c_int, _i32p, C.c_double,
+                                             C.c_double, C.c_double, C.c_double, _f64p, C.c_int]
         L.dx_energy.argtypes = [C.c_int, _f64p, _f64p, _f64p, C.c_double, C.c_double, _f64p,
                                 C.c_int]
         L.dx_moments.argtypes = [C.c_int, _f64p, _f64p, _f64p]
@@ -296,12 +298,13 @@ def direct_fp64(pos, mass, targets, eps, min_d2=0.0, nthreads=0):
     return acc
 
 
-def direct_tpm_fp64(pos, mass, targets, eps, rs, rc, nthreads=0):
+def direct_tpm_fp64(pos, mass, targets, eps, rs, rc, nthreads=0, box=0.0):
     p = np.ascontiguousarray(pos, np.float64).reshape(-1)
     m = np.ascontiguousarray(mass, np.float64)
     t = np.ascontiguousarray(targets, np.int32)
     acc = np.zeros((t.size, 3), np.float64)
-    lib().dx_direct_tpm_fp64(m.size, p, m, t.size, t, eps, rs, rc, acc.reshape(-1), nthreads)
+    lib().dx_direct_tpm_fp64_box(m.size, p, m, t.size, t, eps, rs, rc, box, acc.reshape(-1),
+                                 nthreads)
     return acc
 
 

@@ -53,6 +53,32 @@ def test_create_rejects_bad_arguments_without_device():
     assert b"invalid" in err.value
 
 
+@pytest.mark.parametrize("fields,why", [
+    (dict(periodic_box=1.0), b"short-range"),
+    (dict(periodic_box=1.0, treepm_split_scale=0.1, treepm_cutoff=0.3), b"periodic_box / 4"),
+    (dict(pm_mesh=64), b"pm_mesh needs periodic_box"),
+    (dict(pm_mesh=63, periodic_box=1.0, treepm_split_scale=0.02, treepm_cutoff=0.09), b"even"),
+])
+def test_create_rejects_invalid_periodic_settings_without_device(fields, why):
+    import blitzar_amd
+    from blitzar_amd import _abi
+    h = C.c_void_p()
+    err = C.create_string_buffer(256)
+    p = blitzar_amd.default_params()
+    for k, v in fields.items():
+        setattr(p, k, v)
+    rc = blitzar_amd.lib().bh_hip_create(C.byref(p), 100, C.byref(h), err, 256)
+    assert rc == _abi.INVALID_ARGUMENT and not h.value
+    assert why in err.value
+
+
+def test_params_layout_matches_header():
+    from blitzar_amd import _abi
+    # 16 four-byte configuration fields + 5 block-step / periodic / mesh fields + 2 reserved
+    assert C.sizeof(_abi.Params) == 4 * 23
+    assert C.sizeof(_abi.LffConfig) == 20
+
+
 def test_initial_conditions_deterministic_and_centred():
     import blitzar_amd as B
     p1, v1, m1 = B.plummer(20000, 42)

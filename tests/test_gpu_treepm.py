@@ -39,3 +39,114 @@ def test_short_range_walk(oracle, theta):
     acpu = oracle.octree_forces_subset(x, t, oracle.make_params(
         theta=theta, softening=eps, short_range_scale=rs, cutoff2=rc * rc))
     assert np.median(rel(a[t], acpu)) < 2e-3
+
+
+def _box_case(n, seed):
+    rng = np.random.default_rng(seed)
+    pos = rng.random((n, 3))
+    m = np.full(n, 1.0 / n, np.float32)
+    cell = 1.0 / 64.0
+    rs = 1.25 * cell
+    return pos, m, rs, 4.5 * rs, 0.1 * cell
+
+
+@pytest.mark.parametrize("theta", [0.05, 0.6])
+def test_periodic_short_range_walk(oracle, theta):
+    import blitzar_amd as B
+    n = 40000
+    print(f"case periodic treepm n={n} theta={theta}", flush=True)
+    pos, m, rs, rc, eps = _box_case(n, 13)
+    kw = dict(theta=theta, softening=eps, treepm_split_scale=rs, treepm_cutoff=rc)
+    with B.OctreeHip(n, periodic_box=1.0, **kw) as s:
+        s.upload(pos, None, m)
+        a = s.compute_accelerations()
+    with B.OctreeHip(n, **kw) as s:
+        s.upload(pos, None, m)
+        a_open = s.compute_accelerations()
+    face = np.any((pos < rc) | (pos > 1 - rc), axis=1)
+    t = np.concatenate([np.nonzero(face)[0][::23], np.arange(0, n, 97)])
+    f = lambda x: float(np.float32(x))
+    ref = oracle.direct_tpm_fp64(pos, m.astype(np.float64), t, f(eps), f(rs), f(rc), box=1.0)
+    r = rel(a[t], ref)
+    print(f"  median={np.median(r):.3e} p99={np.percentile(r, 99):.3e}", flush=True)
+    if theta < 0.1:
+        assert np.median(r) < 1e-5
+        assert np.percentile(r, 99) < 1e-3
+    else:
+        assert np.median(r) < 1e-3
+        assert np.percentile(r, 99) < 1e-2
+    # particles near a face feel their neighbours across it
+    tf = np.nonzero(face)[0][:200]
+    assert np.median(rel(a_open[tf], a[tf].astype(np.float64))) > 0.05
+
+
+def test_periodic_translation_invariance(oracle):
+    import blitzar_amd as B
+    n = 20000
+    pos, m, rs, rc, eps = _box_case(n, 14)
+    kw = dict(theta=0.05, softening=eps, treepm_split_scale=rs, treepm_cutoff=rc,
+              periodic_box=1.0)
+    out = []
+    for shift in ((0, 0, 0), (0.37, 0.61, 0.13)):
+        with B.OctreeHip(n, **kw) as s:
+            s.upload(np.mod(pos + np.array(shift), 1.0), None, m)
+            out.append(s.compute_accelerations())
+    r = rel(out[1], out[0].astype(np.float64))
+    assert np.median(r) < 1e-5 and np.percentile(r, 99) < 1e-3
+
+
+def test_periodic_positions_stay_in_the_box():
+    import blitzar_amd as B
+    n = 5000
+    pos, m, rs, rc, eps = _box_case(n, 15)
+    m = np.full(n, 1e-12 / n, np.float32)  # forces negligible
+    vel = np.random.default_rng(1).normal(size=(n, 3)) * 3.0
+    with B.OctreeHip(n, theta=0.6, softening=eps, treepm_split_scale=rs, treepm_cutoff=rc,
+                     periodic_box=1.0) as s:
+        s.upload(pos, vel, m)
+        s.step(0.05, 4)
+        p, v = s.download()
+    assert np.all((p >= 0.0) & (p < 1.0))
+    # the motion is a straight line modulo the box
+    np.testing.assert_allclose(np.mod(pos + 0.2 * v - p + 0.5, 1.0) - 0.5, 0.0, atol=1e-9)
+
+
+def test_periodic_rejects_a_cutoff_beyond_a_quarter_box():
+    import blitzar_amd as B
+    from blitzar_amd.solver import BhHipError
+    with pytest.raises(BhHipError):
+        B.OctreeHip(100, treepm_split_scale=0.1, treepm_cutoff=0.3, periodic_box=1.0)
+    with pytest.raises(BhHipError):
+        B.OctreeHip(100, periodic_box=1.0)  # no short-range split
+
+
+def test_particle_mesh_matches_oracle():
+    import blitzar_amd as B
+    from oracle import pm
+    n, N = 20000, 32
+    print(f"case particle mesh n={n} mesh={N}", flush=True)
+    rng = np.random.default_rng(16)
+    # a clumpy periodic field: half uniform, half in two blobs (one across a face)
+    pos = np.concatenate([rng.random((n // 2, 3)),
+                          np.mod(rng.normal([0.3, 0.5, 0.5], 0.05, (n // 4, 3)), 1.0),
+                          np.mod(rng.normal([0.98, 0.1, 0.7], 0.03, (n // 4, 3)), 1.0)])
+    m = rng.uniform(0.5, 1.5, n).astype(np.float32) / n
+    h = 1.0 / N
+    rs, rc, eps = 1.25 * h, 4.5 * 1.25 * h, 0.1 * h
+    kw = dict(theta=0.6, softening=eps, treepm_split_scale=rs, treepm_cutoff=rc,
+              periodic_box=1.0)
+    with B.OctreeHip(n, pm_mesh=N, **kw) as s:
+        s.upload(pos, None, m)
+        a_pm = s.pm_accelerations()
+        a_tot = s.compute_accelerations()
+    with B.OctreeHip(n, **kw) as s:
+        s.upload(pos, None, m)
+        a_short = s.compute_accelerations()
+    ref = pm.long_range_accel(pos, m.astype(np.float64), 1.0, N, float(np.float32(rs)))
+    r = rel(a_pm, ref)
+    l2 = np.linalg.norm(a_pm - ref) / np.linalg.norm(ref)
+    print(f"  mesh vs FP64 oracle: median={np.median(r):.3e} l2={l2:.3e}", flush=True)
+    assert np.median(r) < 1e-4 and l2 < 1e-4
+    # the force pass is the short-range walk plus the mesh
+    np.testing.assert_allclose(a_tot, a_short + a_pm, rtol=1e-4,
+                               atol=1e-5 * np.abs(a_tot).max())

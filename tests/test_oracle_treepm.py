@@ -38,3 +38,37 @@ def test_octree_short_range_equals_direct(oracle):
                                  eps, rs, rc)
     err = np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
     assert np.median(err) < 1e-5
+
+
+def test_periodic_two_particles_across_the_boundary(oracle):
+    # L = 1: particles at x = 0.01 and x = 0.99 are 0.02 apart through the face
+    rs, rc, eps = 0.05, 0.2, 1e-3
+    pos = np.array([[0.01, 0.5, 0.5], [0.99, 0.5, 0.5]])
+    m = np.array([1.0, 2.0])
+    a = oracle.direct_tpm_fp64(pos, m, np.array([0, 1]), eps, rs, rc, box=1.0)
+    r = math.sqrt(0.02 ** 2 + eps * eps)
+    mag = 0.02 / r ** 3 * w(r, rs)
+    assert a[0, 0] == pytest.approx(-2.0 * mag, rel=1e-9)  # pulled through x = 0
+    assert a[1, 0] == pytest.approx(1.0 * mag, rel=1e-9)
+    assert np.all(np.abs(a[:, 1:]) < 1e-12)
+    # without the box the pair is 0.98 apart: beyond the cutoff
+    a_open = oracle.direct_tpm_fp64(pos, m, np.array([0, 1]), eps, rs, rc)
+    assert np.all(a_open == 0.0)
+
+
+def test_periodic_translation_invariance(oracle):
+    rng = np.random.default_rng(9)
+    n = 2000
+    pos = rng.random((n, 3))
+    m = rng.uniform(0.5, 1.5, n) / n
+    rs, rc, eps = 0.02, 0.09, 0.002
+    t = np.arange(0, n, 7)
+    a = oracle.direct_tpm_fp64(pos, m, t, eps, rs, rc, box=1.0)
+    shifted = np.mod(pos + np.array([0.37, 0.61, 0.13]), 1.0)
+    b = oracle.direct_tpm_fp64(shifted, m, t, eps, rs, rc, box=1.0)
+    np.testing.assert_allclose(b, a, rtol=1e-9, atol=1e-12 * np.abs(a).max())
+    # far from every face the box changes nothing
+    inner = np.arange(n)[np.all((pos > 0.1) & (pos < 0.9), axis=1)][:50]
+    np.testing.assert_array_equal(
+        oracle.direct_tpm_fp64(pos, m, inner, eps, rs, rc, box=1.0),
+        oracle.direct_tpm_fp64(pos, m, inner, eps, rs, rc))
