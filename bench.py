@@ -114,6 +114,61 @@ def cpu_model():
     return "unknown"
 
 
+def zeldovich_lattice(ng, seed=3, rms_cells=0.3, box=1.0):
+    """ng^3 lattice displaced by a first-order Zel'dovich field: Gaussian
+    random density with P(k) ~ k^-2, displacement psi_k = i k delta_k / k^2,
+    scaled to an RMS displacement of rms_cells lattice cells per axis."""
+    rng = np.random.default_rng(seed)
+    white = rng.standard_normal((ng, ng, ng)).astype(np.float32)
+    dk = np.fft.rfftn(white)
+    del white
+    k = np.fft.fftfreq(ng, d=1.0 / ng).astype(np.float32)
+    kz = np.arange(ng // 2 + 1, dtype=np.float32)
+    k2 = k[:, None, None] ** 2 + k[None, :, None] ** 2 + kz[None, None, :] ** 2
+    k2[0, 0, 0] = 1.0
+    dk *= 1.0 / np.sqrt(k2)  # amplitude sqrt(P) ~ 1/k
+    dk[0, 0, 0] = 0.0
+    g = (np.arange(ng, dtype=np.float64) + 0.5) / ng * box
+    pos = np.empty((ng ** 3, 3), np.float64)
+    disp = []
+    for a, K in enumerate((k[:, None, None], k[None, :, None], kz[None, None, :])):
+        comp = np.fft.irfftn(1j * K * dk / k2, s=(ng, ng, ng), axes=(0, 1, 2))
+        disp.append(comp.astype(np.float32).reshape(-1))
+        del comp
+    scale = rms_cells * (box / ng) / np.sqrt(np.mean([np.mean(d * d) for d in disp]))
+    grid = np.meshgrid(g, g, g, indexing="ij")
+    for a in range(3):
+        pos[:, a] = np.mod(grid[a].reshape(-1) + scale * disp[a], box)
+    return pos
+
+
+def periodic_treepm(steps, ng=256):
+    """Periodic TreePM box: ng^3 Zel'dovich lattice (seed 3, rms 0.3 cells),
+    L = 1, rocFFT mesh ng^3, r_s = 1.25 cells, short-range cut at 4.5 r_s,
+    theta 0.6, eps 0.05 cells, KDK dt 1e-3, zero initial velocities."""
+    import blitzar_amd as B
+    t0 = time.perf_counter()
+    pos = zeldovich_lattice(ng)
+    n = pos.shape[0]
+    t_ic = time.perf_counter() - t0
+    h = 1.0 / ng
+    with B.OctreeHip(n, theta=0.6, softening=0.05 * h, treepm_split_scale=1.25 * h,
+                     treepm_cutoff=4.5 * 1.25 * h, periodic_box=1.0, pm_mesh=ng,
+                     integrator="leapfrog") as s:
+        s.set_timing(True)
+        s.upload(pos, None, np.full(n, 1.0 / n, np.float32))
+        s.step(1e-3, 1)
+        t0 = time.perf_counter()
+        s.step(1e-3, steps)
+        el = time.perf_counter() - t0
+        tm = s.timings()
+    walks = max(1, tm["walks"])
+    return {"workload": f"zeldovich_{ng}^3_box1_pm{ng}_rs1.25cells_theta0.6", "n": n,
+            "steps": steps, "value": n * steps / el, "unit": "particle-steps/s",
+            "ms_per_step": el * 1e3 / steps, "ic_seconds": t_ic,
+            "build_ms": tm["build_ms"] / walks, "walk_and_mesh_ms": tm["walk_ms"] / walks}
+
+
 def small_configs(with_cpu):
     """The two smaller reference configurations, each on its own handle:
     Plummer 65,536 (seed 42, eps 0.01, theta 0.5, KDK dt 1e-3, 100 steps, FP64
@@ -261,6 +316,8 @@ def main(argv=None):
     ap.add_argument("--lff", action="store_true")
     ap.add_argument("--block-steps", type=int, default=0,
                     help="also run N base steps with dyadic block time steps")
+    ap.add_argument("--treepm", type=int, default=0,
+                    help="also run N steps of the periodic TreePM box (256^3 Zel'dovich lattice)")
     ap.add_argument("--eps", type=float, default=0.01)
     ap.add_argument("--order", type=int, default=2)
     ap.add_argument("--bucket", type=int, default=16)
@@ -350,6 +407,12 @@ def main(argv=None):
         log(f"[bench] cpu baseline {cpu} ({time.perf_counter() - t0:.1f} s)")
     s.close()
 
+    treepm_line = None
+    if args.treepm > 0:
+        t0 = time.perf_counter()
+        treepm_line = periodic_treepm(args.treepm)
+        log(f"[bench] periodic TreePM ({time.perf_counter() - t0:.1f} s): {treepm_line}")
+
     configs = None
     if not args.no_configs:
         t0 = time.perf_counter()
@@ -398,6 +461,8 @@ def main(argv=None):
         line["block_time_steps"] = block_line
     if configs is not None:
         line["configs"] = configs
+    if treepm_line is not None:
+        line["periodic_treepm"] = treepm_line
     print(json.dumps(line), flush=True)
     return 0
 

@@ -20,7 +20,7 @@
 
 namespace {
 
-constexpr int kDefaultWavesPerCu = 16;
+constexpr int kDefaultWavesPerCu = 24;
 constexpr size_t kStackCap = 16384;
 
 struct StepEvents {

@@ -44,8 +44,17 @@
 namespace bh {
 namespace {
 
-constexpr int kCellCap = 128;
-constexpr int kPartCap = 256;
+// LDS list capacities (cells flush at kCellCap - 64, particles at kPartCap);
+// together with the 64-lane stack window they set the LDS per wavefront and
+// so the walk's residency.
+#ifndef BH_WALK_CELL_CAP
+#define BH_WALK_CELL_CAP 128
+#endif
+#ifndef BH_WALK_PART_CAP
+#define BH_WALK_PART_CAP 128
+#endif
+constexpr int kCellCap = BH_WALK_CELL_CAP;
+constexpr int kPartCap = BH_WALK_PART_CAP;
 
 __device__ inline float wmin(float v)
 {
