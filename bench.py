@@ -94,6 +94,25 @@ def cpu_baseline(pos, m, theta, eps, n_targets, seed=123):
     }
 
 
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "walk_build_merger8M_theta0.8_pmc.md")
+
+
+def walk_traffic(args, n):
+    """HBM bytes per walk launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (2 x FETCH + WRITE, calibrated on the
+    streaming kick/drift kernel), or None for other workloads."""
+    if not (args.workload == "merger" and n == 8388608 and args.theta == 0.8 and args.order == 2):
+        return None
+    try:
+        with open(PMC_SUMMARY) as f:
+            for line in f:
+                if line.startswith("walk_hbm_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def force_error(acc, ref, samples):
     acc = np.asarray(acc, np.float64)
     r = np.linalg.norm(acc - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), 1e-30)
@@ -448,7 +467,8 @@ def main(argv=None):
         "roofline": {
             "kernel": "walk_kernel", "bound": "mfma", "pipe": "fp32 VALU",
             "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+            "frac": achieved / PEAK_FP32_TFLOPS, "traffic": walk_traffic(args, n),
+            "traffic_source": "profiles/walk_build_merger8M_theta0.8_pmc.md (rocprofv3 PMC)",
             "interactions_per_launch": {"pp": tm["pp_interactions"] / walks,
                                         "pc": tm["pc_interactions"] / walks},
         },

@@ -467,33 +467,49 @@ bh_hip_result_t step_adaptive(bh_hip_handle* h, double dt, int32_t nsteps)
     const bh::AdaptiveParams a = adaptive_params(h, dt);
     const bh::WalkParams wp0 = walk_params(h->p);
     if (!h->ad_primed || std::fabs(h->ad_quantum - a.quantum) > 1e-12 * a.quantum) {
+        // ticks restart at 0 so that every cadence 2^(L - level) divides the
+        // start of each global step
+        h->ad_tick = 0;
         StepEvents* ev = next_events(h);
         queue_build(h, ev);
         bh::launch_walk(d, wp0, h->stats);
         rec(ev ? ev->w1 : nullptr, d.stream);
-        bh::launch_adaptive_init(d, a, static_cast<uint32_t>(h->ad_tick));
+        bh::launch_adaptive_init(d, a, 0u);
         h->ad_primed = true;
         h->ad_quantum = a.quantum;
     }
+    // Event-driven slices: from tick t the next tick at which any particle is
+    // due is found on the device; the slices in between have no force
+    // evaluation, so one predictor spans them (constant acceleration makes k
+    // single-slice predictors one predictor over k q).
+    uint32_t* due = reinterpret_cast<uint32_t*>(d.diag);
     const uint64_t slices = 1ull << a.levels;
     for (int k = 0; k < nsteps; ++k) {
-        for (uint64_t sl = 0; sl < slices; ++sl) {
-            const uint32_t t = static_cast<uint32_t>(++h->ad_tick);
-            bh::launch_adaptive_drift(d, a);
+        const uint64_t end = h->ad_tick + slices;
+        while (h->ad_tick < end) {
+            const uint32_t t0 = static_cast<uint32_t>(h->ad_tick);
+            bh::launch_adaptive_next_due(d, a, t0, due);
+            uint32_t next = 0;
+            if (!check(h, hipMemcpyAsync(&next, due, sizeof(next), hipMemcpyDeviceToHost,
+                                         d.stream),
+                       "adaptive schedule") ||
+                !check(h, hipStreamSynchronize(d.stream), "adaptive schedule")) {
+                end_call(h);
+                return BH_HIP_INTERNAL_ERROR;
+            }
+            const uint64_t t = std::min<uint64_t>(end, h->ad_tick + (next - t0));
+            bh::launch_adaptive_drift(d, a, static_cast<double>(t - h->ad_tick) * a.quantum);
+            h->ad_tick = t;
+            const uint32_t tt = static_cast<uint32_t>(t);
             StepEvents* ev = next_events(h);
             queue_build(h, ev);
             bh::WalkParams wp = wp0;
             wp.alevel = d.alevel;
-            wp.atick = t;
+            wp.atick = tt;
             wp.alevels = a.levels;
             bh::launch_walk(d, wp, h->stats);
             rec(ev ? ev->w1 : nullptr, d.stream);
-            bh::launch_adaptive_correct(d, a, t, h->stats + 2);
-        }
-        // keep the host queue bounded on long runs
-        if (!check(h, hipStreamSynchronize(d.stream), "adaptive step")) {
-            end_call(h);
-            return BH_HIP_INTERNAL_ERROR;
+            bh::launch_adaptive_correct(d, a, tt, h->stats + 2);
         }
     }
     h->primed = false;

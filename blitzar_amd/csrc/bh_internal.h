@@ -307,7 +307,11 @@ void pm_release(PmState& pm);
 bool launch_pm_accel(Device& d, PmState& pm, double L, float rs, float max_acc);
 // block time steps
 void launch_adaptive_init(Device& d, const AdaptiveParams& a, uint32_t tick);
-void launch_adaptive_drift(Device& d, const AdaptiveParams& a);
+// Predictor over `span` (k slices of constant acceleration: x += v s + a s^2/2,
+// v += a s, the same as k single-slice predictors).
+void launch_adaptive_drift(Device& d, const AdaptiveParams& a, double span);
+// *out = the first tick after t at which some particle is due.
+void launch_adaptive_next_due(Device& d, const AdaptiveParams& a, uint32_t t, uint32_t* out);
 // count (optional, device): += number of particles corrected
 void launch_adaptive_correct(Device& d, const AdaptiveParams& a, uint32_t tick,
                              unsigned long long* count);

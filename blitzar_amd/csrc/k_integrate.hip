@@ -434,6 +434,25 @@ __global__ __launch_bounds__(256) void adaptive_correct_kernel(
     level[i] = adaptive_level(nx[i], ny[i], nz[i], x, y, z, a);
 }
 
+// Smallest tick after t at which some particle is due: min over particles of
+// (t / cadence + 1) * cadence (ticks count from a multiple of 2^L).
+__global__ __launch_bounds__(256) void adaptive_next_due_kernel(const uint32_t* __restrict__ level,
+                                                                size_t n, int L, uint32_t t,
+                                                                uint32_t* __restrict__ out)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    uint32_t due = 0xffffffffu;
+    if (i < n) {
+        const uint32_t lv = min(level[i], static_cast<uint32_t>(L));
+        const uint32_t c = 1u << (static_cast<uint32_t>(L) - lv);
+        due = (t / c + 1u) * c;
+    }
+    for (int o = 32; o > 0; o >>= 1)
+        due = min(due, static_cast<uint32_t>(__shfl_xor(static_cast<int>(due), o, 64)));
+    if ((threadIdx.x & 63) == 0 && due != 0xffffffffu)
+        atomicMin(out, due);
+}
+
 __global__ __launch_bounds__(256) void gather_u32_kernel(const uint32_t* __restrict__ src,
                                                          const uint32_t* __restrict__ rank,
                                                          size_t n, uint32_t* __restrict__ out)
@@ -454,13 +473,22 @@ void launch_adaptive_init(Device& d, const AdaptiveParams& a, uint32_t tick)
         d.aacc[2], d.alevel, d.atick, d.n, a, tick);
 }
 
-void launch_adaptive_drift(Device& d, const AdaptiveParams& a)
+void launch_adaptive_drift(Device& d, const AdaptiveParams& a, double span)
 {
     if (d.n == 0)
         return;
     adaptive_drift_kernel<<<blocks_for(d.n), 256, 0, d.stream>>>(
         d.pos[0], d.pos[1], d.pos[2], d.vel[0], d.vel[1], d.vel[2], d.aacc[0], d.aacc[1],
-        d.aacc[2], d.n, a.quantum, a.max_speed, a.box);
+        d.aacc[2], d.n, span, a.max_speed, a.box);
+}
+
+void launch_adaptive_next_due(Device& d, const AdaptiveParams& a, uint32_t t, uint32_t* out)
+{
+    launch_fill_u32(out, 0xffffffffu, 1, d.stream);
+    if (d.n == 0)
+        return;
+    adaptive_next_due_kernel<<<blocks_for(d.n), 256, 0, d.stream>>>(d.alevel, d.n, a.levels, t,
+                                                                     out);
 }
 
 void launch_adaptive_correct(Device& d, const AdaptiveParams& a, uint32_t tick,
