@@ -14,9 +14,11 @@ initial conditions (generated on the host, uploaded once).
 
 The opening angle is the accuracy knob of the metric: the headline runs at
 theta = 0.8 (the largest of 0.5, 0.6, ..., 0.9 whose median error stays below
-1e-3 on this state), and the same run also times theta = 0.5, the opening
-angle the workload is usually quoted at, under "theta_0.5". Both force
-errors are measured in the run. `--lff` adds the learned far-field
+1e-3 on this state) with the group far-field expansion (far_alpha 0.75:
+accepted nodes far from a whole target group enter one local expansion per
+group), and the same run also times the plain walk at theta = 0.5, the
+opening angle the workload is usually quoted at, under "theta_0.5". Both
+force errors are measured in the run. `--lff` adds the learned far-field
 correction (theta 0.9, trained on a second realisation, seed 12).
 
 Prints one JSON line (see the keys below). `--gpus N` with N > 1 exits: this
@@ -40,6 +42,8 @@ PEAK_FP32_TFLOPS = 157.3
 FLOP_PP = 20.0   # particle-particle interaction
 FLOP_PC = 50.0   # particle-cell interaction, monopole + quadrupole
 FLOP_PC_MONO = 20.0
+FLOP_FAR_NODE = 180.0  # node -> group local expansion (monopole to 2nd, quadrupole to 1st order)
+FLOP_FAR_EVAL = 48.0   # evaluation of the group expansion at one target
 
 
 def log(*a):
@@ -101,7 +105,8 @@ def walk_traffic(args, n):
     """HBM bytes per walk launch from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this workload (2 x FETCH + WRITE, calibrated on the
     streaming kick/drift kernel), or None for other workloads."""
-    if not (args.workload == "merger" and n == 8388608 and args.theta == 0.8 and args.order == 2):
+    if not (args.workload == "merger" and n == 8388608 and args.theta == 0.8 and args.order == 2
+            and args.far_alpha == 0.75):
         return None
     try:
         with open(PMC_SUMMARY) as f:
@@ -330,6 +335,8 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=8388608)
     ap.add_argument("--workload", default="merger", choices=["merger", "plummer"])
     ap.add_argument("--theta", type=float, default=0.8)
+    ap.add_argument("--far-alpha", type=float, default=0.75,
+                    help="group far-field expansion (0 = plain walk)")
     ap.add_argument("--ref-theta", type=float, default=0.5)
     ap.add_argument("--ref-steps", type=int, default=5)
     ap.add_argument("--lff", action="store_true")
@@ -362,7 +369,7 @@ def main(argv=None):
     log(f"[bench] state {args.workload} n={n} generated in {time.perf_counter() - t0:.2f} s")
 
     s = B.OctreeHip(n, theta=args.theta, softening=args.eps, multipole_order=args.order,
-                    bucket_capacity=args.bucket, integrator="leapfrog")
+                    bucket_capacity=args.bucket, integrator="leapfrog", far_alpha=args.far_alpha)
     s.upload(pos, vel, m)
     s.set_timing(True)
     s.compute_accelerations(to_host=False)
@@ -379,7 +386,8 @@ def main(argv=None):
     walks = max(1, tm["walks"])
     walk_ms = tm["walk_ms"] / walks
     flop_pc = FLOP_PC if args.order >= 2 else FLOP_PC_MONO
-    flop = (FLOP_PP * tm["pp_interactions"] + flop_pc * tm["pc_interactions"]) / walks
+    flop = (FLOP_PP * tm["pp_interactions"] + flop_pc * tm["pc_interactions"] +
+            FLOP_FAR_NODE * tm["far_interactions"] + FLOP_FAR_EVAL * tm["far_evaluations"]) / walks
     achieved = flop / (walk_ms * 1e-3) / 1e12
 
     err = None
@@ -395,7 +403,7 @@ def main(argv=None):
         pos = p_now
 
     # the same state at the reference opening angle (timed the same way)
-    s.set_params(theta=args.ref_theta)
+    s.set_params(theta=args.ref_theta, far_alpha=0.0)
     s.step(args.dt, 1)
     t0 = time.perf_counter()
     s.step(args.dt, args.ref_steps)
@@ -454,8 +462,10 @@ def main(argv=None):
         "data": "synthetic",
         "config": {
             "workload": (f"{'hernquist_merger' if args.workload == 'merger' else 'plummer'}"
-                         f"_{n}_theta{args.theta}_{'quadrupole' if args.order >= 2 else 'monopole'}"),
-            "n": n, "theta": args.theta, "softening": args.eps, "dt": args.dt,
+                         f"_{n}_theta{args.theta}_{'quadrupole' if args.order >= 2 else 'monopole'}"
+                         f"{f'_far{args.far_alpha}' if args.far_alpha > 0 else ''}"),
+            "n": n, "theta": args.theta, "far_alpha": args.far_alpha, "softening": args.eps,
+            "dt": args.dt,
             "integrator": "leapfrog_kdk_fp64", "bucket": args.bucket,
             "parallelism": "none (1 GPU)",
         },
@@ -470,7 +480,9 @@ def main(argv=None):
             "frac": achieved / PEAK_FP32_TFLOPS, "traffic": walk_traffic(args, n),
             "traffic_source": "profiles/walk_build_merger8M_theta0.8_pmc.md (rocprofv3 PMC)",
             "interactions_per_launch": {"pp": tm["pp_interactions"] / walks,
-                                        "pc": tm["pc_interactions"] / walks},
+                                        "pc": tm["pc_interactions"] / walks,
+                                        "far_node_group": tm["far_interactions"] / walks,
+                                        "far_evaluations": tm["far_evaluations"] / walks},
         },
         "cpu_baseline": cpu,
         f"theta_{args.ref_theta}": ref_line,

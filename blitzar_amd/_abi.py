@@ -41,7 +41,8 @@ class Params(C.Structure):
         ("adaptive_flags", C.c_int32),
         ("periodic_box", C.c_float),
         ("pm_mesh", C.c_int32),
-        ("reserved", C.c_int32 * 2),
+        ("far_alpha", C.c_float),
+        ("reserved", C.c_int32 * 1),
     ]
 
 
@@ -59,6 +60,8 @@ class Timings(C.Structure):
         ("walks", C.c_int32),
         ("groups", C.c_int32),
         ("particle_steps", C.c_uint64),
+        ("far_interactions", C.c_uint64),
+        ("far_evaluations", C.c_uint64),
     ]
 
 

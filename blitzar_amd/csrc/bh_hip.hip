@@ -120,6 +120,7 @@ bh_hip_params_t resolve(const bh_hip_params_t& in)
     p.adaptive_flags = in.adaptive_flags & BH_HIP_ADAPTIVE_VELOCITY;
     p.periodic_box = std::max(0.0f, in.periodic_box);
     p.pm_mesh = std::max(0, in.pm_mesh);
+    p.far_alpha = std::clamp(in.far_alpha, 0.0f, 1.0f);
     if (p.waves_per_cu <= 0)
         p.waves_per_cu = kDefaultWavesPerCu;
     return p;
@@ -208,7 +209,7 @@ bool alloc_all(bh_hip_handle* h, uint64_t n)
     if (!dalloc(h, &d.level_info, 2 * bh::kLevels + 8, "level info") ||
         !dalloc(h, &d.geom, 1, "geometry") || !dalloc(h, &d.aabb_bits, 8, "aabb") ||
         !dalloc(h, &d.err, 4, "error flags") || !dalloc(h, &d.work_counter, 4, "work counter") ||
-        !dalloc(h, &h->stats, 3, "walk stats"))
+        !dalloc(h, &h->stats, 5, "walk stats"))
         return false;
     d.diag_cap = 6 * 1024 + 8 * 256 + 16;  // AABB partials, energy partials
     if (!dalloc(h, &d.diag, d.diag_cap, "diagnostics"))
@@ -322,6 +323,7 @@ bh::WalkParams walk_params(const bh_hip_params_t& p)
     w.rs = p.treepm_split_scale;
     w.rcut2 = p.treepm_cutoff > 0.0f ? p.treepm_cutoff * p.treepm_cutoff : 0.0f;
     w.box = p.periodic_box;
+    w.far2 = p.far_alpha > 0.0f ? p.far_alpha * p.far_alpha : 0.0f;
     return w;
 }
 
@@ -372,7 +374,7 @@ void begin_call(bh_hip_handle* h)
 {
     h->events_used = 0;
     h->integrate_ms = 0.0;
-    bh::launch_fill_u32(reinterpret_cast<uint32_t*>(h->stats), 0u, 6, h->d.stream);
+    bh::launch_fill_u32(reinterpret_cast<uint32_t*>(h->stats), 0u, 10, h->d.stream);
 }
 
 // After a synchronised call: fold event times and counters into h->t.
@@ -392,11 +394,13 @@ void end_call(bh_hip_handle* h)
         }
         t.integrate_ms = h->integrate_ms;
     }
-    unsigned long long st[3] = {0, 0, 0};
+    unsigned long long st[5] = {0, 0, 0, 0, 0};
     if (hipMemcpy(st, h->stats, sizeof(st), hipMemcpyDeviceToHost) == hipSuccess) {
         t.pp_interactions = st[0];
         t.pc_interactions = st[1];
         t.particle_steps = st[2];
+        t.far_interactions = st[3];
+        t.far_evaluations = st[4];
     }
     int info[2 * bh::kLevels + 4];
     if (hipMemcpy(info, h->d.level_info, sizeof(info), hipMemcpyDeviceToHost) == hipSuccess) {

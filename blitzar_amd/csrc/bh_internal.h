@@ -110,6 +110,7 @@ struct WalkParams {
     float rs;           // TreePM split scale (0 = Newton)
     float rcut2;        // TreePM cutoff squared (0 = none)
     float box = 0.0f;   // periodic cube side for the TreePM walk (0 = open)
+    float far2 = 0.0f;  // group far field: (bmax + group radius)^2 < far2 * d^2 (0 = off)
     // Block time steps: when alevel is set, only groups holding a particle
     // active at tick `atick` are walked (active: atick mod 2^(alevels -
     // level) == 0); the other groups keep their previous acc.
@@ -261,7 +262,8 @@ void launch_tree_moments(Device& d);
 // together; see node_groups in k_tree.hip).
 void launch_groups(Device& d);
 // k_walk.hip
-// stats (optional, device): [0] particle-particle, [1] particle-cell interactions
+// stats (optional, device): [0] particle-particle, [1] particle-cell interactions,
+// [3] node-group far-field interactions, [4] far-field target evaluations
 void launch_walk(Device& d, const WalkParams& p, unsigned long long* stats);
 // Walk of the learned correction. A node that passes the opening test and is
 // near the group, size^2 >= band^2 * (distance from its COM to the group

@@ -167,7 +167,83 @@ __device__ inline void eval_parts(const float4* __restrict__ pl, int np, float t
     }
 }
 
-template <int ORDER, int CRIT, bool TPM, bool REC>
+// Group far field: the local (Taylor) expansion about the group centre of an
+// accepted node's softened second-order field,
+//   a(g + delta) = A + J delta + 1/2 H delta delta,
+// r = COM - g, R^2 = |r|^2 + eps^2, S = (Q + trs I)/3:
+//   A_i  = M r_i R^-3 + 7.5 (rSr) r_i R^-7 - 3 (Sr)_i R^-5 - 1.5 trs r_i R^-5
+//   J_il = M (3 r_i r_l R^-5 - d_il R^-3)
+//          + (52.5 rSr R^-9 - 7.5 trs R^-7) r_i r_l - 15 R^-7 ((Sr)_i r_l + (Sr)_l r_i)
+//          + d_il (1.5 trs R^-5 - 7.5 rSr R^-7) + 3 R^-5 S_il
+//   H_ijk = M (15 r_i r_j r_k R^-7 - 3 (d_ij r_k + d_ik r_j + d_jk r_i) R^-5)
+// (the monopole to second order in delta, the quadrupole to first).
+// Storage: F[0..2] A, F[3..8] J (xx xy xz yy yz zz), F[9..18] H (xxx xxy xxz xyy
+// xyz xzz yyy yyz yzz zzz).
+constexpr int kFarTerms = 19;
+#ifndef BH_FAR_GROUP_WEIGHT
+#define BH_FAR_GROUP_WEIGHT 3.0f
+#endif
+constexpr float kFarGroupWeight = BH_FAR_GROUP_WEIGHT;
+
+__device__ inline void far_accumulate(float rx, float ry, float rz, const WalkNode& w, float eps2,
+                                      float* F)
+{
+    const float R2 = rx * rx + ry * ry + rz * rz + eps2;
+    const float ri = __frsqrt_rn(R2);
+    const float ri2 = ri * ri;
+    const float ri3 = ri * ri2, ri5 = ri3 * ri2, ri7 = ri5 * ri2, ri9 = ri7 * ri2;
+    const float M = w.mass, t = w.trs;
+    const float third = 1.0f / 3.0f;
+    const float sxx = (w.qxx + t) * third, syy = (w.qyy + t) * third;
+    const float szz = (-w.qxx - w.qyy + t) * third;
+    const float sxy = w.qxy * third, sxz = w.qxz * third, syz = w.qyz * third;
+    const float qx = sxx * rx + sxy * ry + sxz * rz;
+    const float qy = sxy * rx + syy * ry + syz * rz;
+    const float qz = sxz * rx + syz * ry + szz * rz;
+    const float rsr = rx * qx + ry * qy + rz * qz;
+    // A
+    const float am = M * ri3 + 7.5f * rsr * ri7 - 1.5f * t * ri5;
+    F[0] += am * rx - 3.0f * ri5 * qx;
+    F[1] += am * ry - 3.0f * ri5 * qy;
+    F[2] += am * rz - 3.0f * ri5 * qz;
+    // J
+    const float jr = 3.0f * M * ri5 + 52.5f * rsr * ri9 - 7.5f * t * ri7;
+    const float jd = -M * ri3 + 1.5f * t * ri5 - 7.5f * rsr * ri7;
+    const float js = 3.0f * ri5, jq = -15.0f * ri7;
+    F[3] += jr * rx * rx + jq * 2.0f * qx * rx + jd + js * sxx;
+    F[4] += jr * rx * ry + jq * (qx * ry + qy * rx) + js * sxy;
+    F[5] += jr * rx * rz + jq * (qx * rz + qz * rx) + js * sxz;
+    F[6] += jr * ry * ry + jq * 2.0f * qy * ry + jd + js * syy;
+    F[7] += jr * ry * rz + jq * (qy * rz + qz * ry) + js * syz;
+    F[8] += jr * rz * rz + jq * 2.0f * qz * rz + jd + js * szz;
+    // H
+    const float c7 = 15.0f * M * ri7, c5 = 3.0f * M * ri5;
+    F[9] += (c7 * rx * rx - 3.0f * c5) * rx;
+    F[10] += (c7 * rx * rx - c5) * ry;
+    F[11] += (c7 * rx * rx - c5) * rz;
+    F[12] += (c7 * ry * ry - c5) * rx;
+    F[13] += c7 * rx * ry * rz;
+    F[14] += (c7 * rz * rz - c5) * rx;
+    F[15] += (c7 * ry * ry - 3.0f * c5) * ry;
+    F[16] += (c7 * ry * ry - c5) * rz;
+    F[17] += (c7 * rz * rz - c5) * ry;
+    F[18] += (c7 * rz * rz - 3.0f * c5) * rz;
+}
+
+__device__ inline void far_evaluate(const float* F, float dx, float dy, float dz, float& ax,
+                                    float& ay, float& az)
+{
+    const float xx = dx * dx, yy = dy * dy, zz = dz * dz;
+    const float xy = dx * dy, xz = dx * dz, yz = dy * dz;
+    ax += F[0] + F[3] * dx + F[4] * dy + F[5] * dz +
+          0.5f * (F[9] * xx + F[12] * yy + F[14] * zz) + F[10] * xy + F[11] * xz + F[13] * yz;
+    ay += F[1] + F[4] * dx + F[6] * dy + F[7] * dz +
+          0.5f * (F[10] * xx + F[15] * yy + F[17] * zz) + F[12] * xy + F[13] * xz + F[16] * yz;
+    az += F[2] + F[5] * dx + F[7] * dy + F[8] * dz +
+          0.5f * (F[11] * xx + F[16] * yy + F[18] * zz) + F[13] * xy + F[14] * xz + F[17] * yz;
+}
+
+template <int ORDER, int CRIT, bool TPM, bool REC, bool FAR>
 __global__ __launch_bounds__(64) void walk_kernel(
     WalkParams P, const WalkNode* __restrict__ nodes, const float4* __restrict__ spart,
     const uint32_t* __restrict__ sleaf, const RootGeom* __restrict__ geom,
@@ -238,6 +314,11 @@ __global__ __launch_bounds__(64) void walk_kernel(
         const float ghx = 0.5f * (gx1 - gx0), ghy = 0.5f * (gy1 - gy0), ghz = 0.5f * (gz1 - gz0);
 
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
+        float F[FAR ? kFarTerms : 1];
+        for (int k = 0; k < (FAR ? kFarTerms : 1); ++k)
+            F[k] = 0.0f;
+        const float grad = FAR ? sqrtf(ghx * ghx + ghy * ghy + ghz * ghz) : 0.0f;
+        unsigned long long nfar = 0;
         int ncell = 0, npart = 0;
         int nrec = 0;
         int sp = 1;
@@ -357,6 +438,22 @@ __global__ __launch_bounds__(64) void walk_kernel(
                 nrec += static_cast<int>(__popcll(rm));
             }
 
+            if (FAR && accept) {
+                // far from the whole group: into this lane's share of the
+                // group's local expansion instead of the per-target list.
+                // The group radius weighs kFarGroupWeight times the node's:
+                // the expansion is second order in the offset from the group
+                // centre, with a larger error constant than the node's own
+                // second-order multipole.
+                const float rx = px - gcx, ry = py - gcy, rz = pz - gcz;
+                const float lim = w.bmax + kFarGroupWeight * grad;
+                if (lim * lim < P.far2 * (rx * rx + ry * ry + rz * rz)) {
+                    far_accumulate(rx, ry, rz, w, P.eps2, F);
+                    accept = false;
+                    ++nfar;
+                }
+            }
+
             // 1) accepted nodes -> LDS cell list
             const unsigned long long am = __ballot(accept);
             if (accept) {
@@ -439,12 +536,28 @@ __global__ __launch_bounds__(64) void walk_kernel(
         eval_parts<TPM>(pl, npart, tx, ty, tz, P, ax, ay, az);
         npc += static_cast<unsigned long long>(ncell);
         npp += static_cast<unsigned long long>(npart);
+        if (FAR) {
+            // sum the lanes' shares of the expansion, then evaluate it per target
+            for (int k = 0; k < kFarTerms; ++k) {
+                float v = F[k];
+                for (int o = 32; o > 0; o >>= 1)
+                    v += __shfl_xor(v, o, 64);
+                F[k] = v;
+            }
+            far_evaluate(F, tx - gcx, ty - gcy, tz - gcz, ax, ay, az);
+            for (int o = 32; o > 0; o >>= 1)
+                nfar += __shfl_xor(nfar, o, 64);
+        }
         // per-target interaction counts for the roofline (only active lanes are real targets)
         npp = npp * static_cast<unsigned long long>(nact);
         npc = npc * static_cast<unsigned long long>(nact);
         if (stats && lane == 0) {
             atomicAdd(&stats[0], npp);
             atomicAdd(&stats[1], npc);
+            if (FAR) {
+                atomicAdd(&stats[3], nfar);                                        // node-group
+                atomicAdd(&stats[4], static_cast<unsigned long long>(nact));      // evaluations
+            }
         }
         npp = 0;
         npc = 0;
@@ -468,11 +581,11 @@ __global__ __launch_bounds__(64) void walk_kernel(
     }
 }
 
-template <int ORDER, int CRIT, bool TPM, bool REC = false>
+template <int ORDER, int CRIT, bool TPM, bool REC = false, bool FAR = false>
 void launch_walk_t(Device& d, const WalkParams& p, unsigned grid, unsigned long long* stats,
                    const LffState* L = nullptr, float band = 0.0f, int min_count = 0)
 {
-    walk_kernel<ORDER, CRIT, TPM, REC><<<grid, 64, 0, d.stream>>>(
+    walk_kernel<ORDER, CRIT, TPM, REC, FAR><<<grid, 64, 0, d.stream>>>(
         p, d.wnode, d.spart, d.sleaf, d.geom, d.groups, d.level_info,
         static_cast<int>(d.node_cap), d.work_counter,
         d.stack, static_cast<int>(d.stack_cap), d.acc[0], d.acc[1], d.acc[2], d.err, stats,
@@ -505,6 +618,16 @@ void launch_walk(Device& d, const WalkParams& p, unsigned long long* stats)
     launch_fill_u32(d.work_counter, 0u, 1, d.stream);
     const unsigned grid = static_cast<unsigned>(d.stack_waves);
     const bool tpm = p.rs > 0.0f;
+    if (p.far2 > 0.0f && !tpm && p.order >= 2) {
+        // quadrupole walk with the group far-field expansion
+        if (p.criterion == 1)
+            launch_walk_t<2, 1, false, false, true>(d, p, grid, stats);
+        else if (p.criterion == 2)
+            launch_walk_t<2, 2, false, false, true>(d, p, grid, stats);
+        else
+            launch_walk_t<2, 0, false, false, true>(d, p, grid, stats);
+        return;
+    }
     if (p.criterion == 2) {
         if (p.order >= 2)
             tpm ? launch_walk_t<2, 2, true>(d, p, grid, stats)

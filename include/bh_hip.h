@@ -87,7 +87,13 @@ typedef struct bh_hip_params {
      * cells per side (0 = short range only). Needs periodic_box and the
      * short-range split; even, 8..1024. */
     int32_t pm_mesh;
-    int32_t reserved[2];
+    /* Group far field (no counterpart in the reference): an accepted node with
+     * (bmax + 3 group radius) < far_alpha * distance from the group centre enters
+     * a second-order local expansion about the group centre, evaluated once
+     * per target, instead of being evaluated per target. 0 = off. Quadrupole
+     * walk without TreePM only. */
+    float far_alpha;
+    int32_t reserved[1];
 } bh_hip_params_t;
 
 /* adaptive_flags: also bound the step by eta * eps / |v| (the reference does) */
@@ -111,6 +117,8 @@ typedef struct bh_hip_timings {
     int32_t walks;            /* force evaluations the sums above cover */
     int32_t groups;           /* walk groups (target sets) of the last tree */
     uint64_t particle_steps;  /* block time steps: particles corrected (sum over slices) */
+    uint64_t far_interactions;  /* group far field: node-group expansions (sum over walks) */
+    uint64_t far_evaluations;   /* group far field: per-target expansion evaluations */
 } bh_hip_timings_t;
 
 /* Fill `out` with the defaults (reference config defaults, CfgConfig.hpp). */

@@ -218,3 +218,31 @@ def test_handles_coexist():
         y1 = s1.compute_accelerations()
     assert np.array_equal(x1, y1)
     assert x2.shape == (5000, 3)
+
+
+@pytest.mark.parametrize("theta,alpha,factor", [(0.5, 0.6, 1.5), (0.5, 0.9, 2.5),
+                                                (0.8, 0.75, 1.2)])
+def test_group_far_field_accuracy(theta, alpha, factor):
+    """Accepted nodes far from the whole group enter the group's local
+    expansion; the forces keep the walk's accuracy against the FP64 sum."""
+    import blitzar_amd as B
+    n = 60000
+    say(f"case far field n={n} theta={theta} alpha={alpha}")
+    pos, vel, m = B.plummer(n, 7)
+    t = np.arange(0, n, 29, dtype=np.uint32)
+    with B.OctreeHip(n, theta=theta, softening=0.005) as s:
+        s.set_timing(True)
+        s.upload(pos, vel, m)
+        ref = s.direct_forces_fp64(t)
+        a0 = s.compute_accelerations()
+        s.set_params(far_alpha=alpha)
+        a1 = s.compute_accelerations()
+        tm = s.timings()
+    e0, e1 = rel(a0[t], ref), rel(a1[t], ref)
+    say(f"  plain median {np.median(e0):.2e} p99 {np.percentile(e0, 99):.2e} | far field "
+        f"median {np.median(e1):.2e} p99 {np.percentile(e1, 99):.2e} "
+        f"far nodes {tm['far_interactions']} cells {tm['pc_interactions']}")
+    assert tm["far_interactions"] > 0
+    # far_alpha trades accuracy for speed: bounded relative to the plain walk
+    assert np.median(e1) < factor * np.median(e0) + 1e-5
+    assert np.percentile(e1, 99) < 2 * factor * np.percentile(e0, 99) + 1e-4

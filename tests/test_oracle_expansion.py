@@ -1,0 +1,66 @@
+"""Pins the FP64 local expansion of a softened node field (oracle/expansion.py)
+by convergence orders against the direct sum, harmonic structure, symmetry and
+agreement with the walk's own second-order cell formula."""
+import numpy as np
+import pytest
+
+from oracle import expansion as ex
+from oracle import mlp
+
+
+def _cluster(seed, n=30, size=1.0):
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-0.5, 0.5, (n, 3)) * size
+    m = rng.uniform(0.5, 1.5, n)
+    M, com, q, trs = mlp.node_moments(pos, m)
+    return pos, m, M, com, q, trs
+
+
+@pytest.mark.parametrize("eps", [0.0, 0.3])
+def test_single_mass_taylor_error_is_third_order(eps):
+    r = np.array([3.0, -1.0, 2.0])
+    A, J, H = ex.local_expansion(1.7, r, np.zeros(6), 0.0, eps)
+    dirn = np.array([0.2, 0.7, -0.68])
+    errs = []
+    for h in (0.2, 0.1, 0.05):
+        x = h * dirn
+        exact = ex.direct(x, r[None, :], [1.7], eps)
+        errs.append(np.linalg.norm(ex.evaluate(A, J, H, x) - exact))
+    assert 7.0 < errs[0] / errs[1] < 9.0 and 7.0 < errs[1] / errs[2] < 9.0
+
+
+def test_centre_value_is_the_walk_cell_formula():
+    pos, m, M, com, q, trs = _cluster(1)
+    x0 = com + np.array([4.0, 1.0, -2.5])
+    for eps in (0.0, 0.4):
+        A, _, _ = ex.local_expansion(M, com - x0, q, trs, eps)
+        want = mlp.multipole_accel(x0, com, M, q, eps, trs)[0]
+        np.testing.assert_allclose(A, want, rtol=1e-12)
+
+
+def test_harmonic_and_symmetric_without_softening():
+    pos, m, M, com, q, trs = _cluster(2)
+    A, J, H = ex.local_expansion(M, np.array([3.0, 2.0, -4.0]), q, trs, 0.0)
+    assert abs(np.trace(J)) < 1e-12 * np.abs(J).max()  # div a = 0 outside the sources
+    np.testing.assert_allclose(J, J.T, rtol=1e-12, atol=1e-15)
+    for perm in ((1, 0, 2), (0, 2, 1), (2, 1, 0)):
+        np.testing.assert_allclose(H, np.transpose(H, perm), atol=1e-15)
+    np.testing.assert_allclose(np.einsum("ijj->i", H), 0.0, atol=1e-12 * np.abs(H).max())
+
+
+def test_cluster_field_error_orders():
+    # halving the group offset delta: the local Taylor error falls ~8x; moving the
+    # cluster twice as far (same delta / d): the whole error falls ~2^4 x
+    pos, m, M, com, q, trs = _cluster(3, size=0.5)
+    dirn = np.array([0.6, -0.64, 0.48])
+    d = 6.0
+    x0 = com + d * dirn
+    A, J, H = ex.local_expansion(M, com - x0, q, trs, 0.0)
+    probe = np.array([0.3, 0.2, -0.25])
+    errs = []
+    for h in (1.0, 0.5, 0.25):
+        x = x0 + h * probe
+        errs.append(np.linalg.norm(ex.evaluate(A, J, H, h * probe)[0] - ex.direct(x, pos, m, 0.0)[0]))
+    assert errs[1] < errs[0] and errs[2] < errs[1]
+    rel = errs[0] / np.linalg.norm(ex.direct(x0, pos, m, 0.0)[0])
+    assert rel < 3e-3
