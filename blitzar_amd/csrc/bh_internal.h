@@ -174,6 +174,7 @@ struct Device {
     uint4* groups = nullptr;          // walk groups: first particle, count, node
 
     uint32_t* scan_tmp = nullptr;  // per-level scan scratch
+    uint32_t* bnd_tmp = nullptr;   // per-level child bounds (8 per node) from count to emit
     uint32_t* scan_blocks = nullptr;
     int* level_info = nullptr;  // [0..22) base, [22..44) count, [44] total, [45] max level, [46] groups
 

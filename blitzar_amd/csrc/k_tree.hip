@@ -148,8 +148,9 @@ __device__ inline void wave_child_bounds(const unsigned long long* __restrict__ 
 __global__ __launch_bounds__(256) void tree_count_kernel(
     const unsigned long long* __restrict__ keys, const uint32_t* __restrict__ nstart,
     const uint32_t* __restrict__ ncount, uint32_t* __restrict__ nchild,
-    uint32_t* __restrict__ nfirst, uint32_t* __restrict__ scan_tmp, int* __restrict__ info,
-    int level, int leaf_depth, uint32_t bucket)
+    uint32_t* __restrict__ nfirst, uint32_t* __restrict__ scan_tmp,
+    uint32_t* __restrict__ bnd_tmp, int* __restrict__ info, int level, int leaf_depth,
+    uint32_t bucket)
 {
     const int base = level_base(info)[level];
     const int cnt = level_count(info)[level];
@@ -170,6 +171,12 @@ __global__ __launch_bounds__(256) void tree_count_kernel(
                 child_bounds(keys, a, a + c, shift, bnd);
             for (int o = 0; o < 8; ++o)
                 nc += bnd[o + 1] > bnd[o] ? 1u : 0u;
+            if (!wave_mode || (threadIdx.x & 63) == 0) {
+                // bounds 1..8 for the emit pass (bound 0 is the node start)
+                uint4* out = reinterpret_cast<uint4*>(bnd_tmp + static_cast<size_t>(k) * 8);
+                out[0] = make_uint4(bnd[1], bnd[2], bnd[3], bnd[4]);
+                out[1] = make_uint4(bnd[5], bnd[6], bnd[7], bnd[8]);
+            }
         }
         if (!wave_mode || (threadIdx.x & 63) == 0) {
             nchild[node] = nc;
@@ -202,43 +209,35 @@ __global__ void tree_level_fix_kernel(int* info, int level, int node_cap, int* e
     info[2 * kLevels] = nb + c;             // total nodes
 }
 
+// Children of the nodes of `level` from the bounds tree_count_kernel left in
+// bnd_tmp (one thread per node: no key reads).
 __global__ __launch_bounds__(256) void tree_emit_kernel(
-    const unsigned long long* __restrict__ keys, uint64_t* __restrict__ nprefix,
-    uint8_t* __restrict__ nlevel, uint32_t* __restrict__ nstart, uint32_t* __restrict__ ncount,
-    uint32_t* __restrict__ nchild, uint32_t* __restrict__ nfirst, uint32_t* __restrict__ nparent,
-    const uint32_t* __restrict__ scan_tmp, const int* __restrict__ info, int level)
+    uint64_t* __restrict__ nprefix, uint8_t* __restrict__ nlevel, uint32_t* __restrict__ nstart,
+    uint32_t* __restrict__ ncount, uint32_t* __restrict__ nchild, uint32_t* __restrict__ nfirst,
+    uint32_t* __restrict__ nparent, const uint32_t* __restrict__ scan_tmp,
+    const uint32_t* __restrict__ bnd_tmp, const int* __restrict__ info, int level)
 {
     const int base = info[level];
     const int cnt = info[kLevels + level];
     const int next_base = info[level + 1];
     const bool dropped = info[kLevels + level + 1] == 0;
-    const int shift = 3 * (kMaxLevel - level - 1);
-    const bool wave_mode = cnt <= kWavePerNodeMax;  // uniform across the grid
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int stride = wave_mode ? (gridDim.x * blockDim.x) >> 6 : gridDim.x * blockDim.x;
-    for (int k = wave_mode ? tid >> 6 : tid; k < cnt; k += stride) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
         const int node = base + k;
         const uint32_t nc = nchild[node];
         if (nc == 0)
-            continue;  // uniform across the wave in wave mode
-        const bool lead = !wave_mode || (threadIdx.x & 63) == 0;
+            continue;
         if (dropped) {
-            if (lead)
-                nchild[node] = 0;
+            nchild[node] = 0;
             continue;
         }
         const uint32_t first = static_cast<uint32_t>(next_base) + scan_tmp[k];
-        const uint32_t a = nstart[node], c = ncount[node];
-        uint32_t bnd[9];
-        if (wave_mode)
-            wave_child_bounds(keys, a, c, shift, bnd);
-        else
-            child_bounds(keys, a, a + c, shift, bnd);
-        if (!lead)
-            continue;
+        const uint4* in = reinterpret_cast<const uint4*>(bnd_tmp + static_cast<size_t>(k) * 8);
+        const uint4 b0 = in[0], b1 = in[1];
+        const uint32_t bnd[9] = {nstart[node], b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
         nfirst[node] = first;
         const unsigned long long p = nprefix[node];
         uint32_t j = 0;
+#pragma unroll
         for (int o = 0; o < 8; ++o) {
             if (bnd[o + 1] <= bnd[o])
                 continue;
@@ -487,14 +486,14 @@ void launch_tree_build(Device& d, int leaf_depth, int bucket)
         const size_t bound = level_bound(L, d.n, d.node_cap);
         const unsigned g = level_grid(bound * 64);  // a wavefront per node on small levels
         tree_count_kernel<<<g, 256, 0, d.stream>>>(keys, d.nstart, d.ncount, d.nchild, d.nfirst,
-                                                   d.scan_tmp, d.level_info, L, leaf_depth,
-                                                   static_cast<uint32_t>(bucket));
+                                                   d.scan_tmp, d.bnd_tmp, d.level_info, L,
+                                                   leaf_depth, static_cast<uint32_t>(bucket));
         launch_scan_exclusive(d.scan_tmp, bound, d.level_info + kLevels + L, 0, d.scan_blocks,
                               nullptr, d.level_info + kLevels + L + 1, d.stream);
         tree_level_fix_kernel<<<1, 64, 0, d.stream>>>(d.level_info, L, cap, d.err);
-        tree_emit_kernel<<<g, 256, 0, d.stream>>>(keys, d.nprefix, d.nlevel, d.nstart, d.ncount,
-                                                  d.nchild, d.nfirst, d.nparent, d.scan_tmp,
-                                                  d.level_info, L);
+        tree_emit_kernel<<<level_grid(bound), 256, 0, d.stream>>>(
+            d.nprefix, d.nlevel, d.nstart, d.ncount, d.nchild, d.nfirst, d.nparent, d.scan_tmp,
+            d.bnd_tmp, d.level_info, L);
     }
 }
 
