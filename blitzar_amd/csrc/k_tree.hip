@@ -383,11 +383,22 @@ __global__ __launch_bounds__(256) void tree_moments_kernel(
 //   * i holds > G particles and has children: its children holding <= G
 //     particles, packed greedily in octant order into runs of consecutive
 //     children with <= G particles in total (a child holding > G ends a run).
+//     A run also ends before a child that would stretch the run's box (the
+//     smallest box of octants covering it) over a sibling holding > G
+//     particles: octants 3 and 4, for instance, span the whole cell, and a
+//     group whose targets surround a dense sibling would open all of it.
 // Every particle lands in exactly one group; a group's particles are
 // contiguous in sorted order and lie inside node i's cell, whose lattice
 // centre is the group's reference point. G = kGroupMax.
+__device__ inline bool octant_box_holds(uint32_t ax, uint32_t ay, uint32_t az, uint32_t o)
+{
+    // ax/ay/az: bit v set when the box covers value v of that axis' octant bit
+    return ((ax >> ((o >> 2) & 1u)) & (ay >> ((o >> 1) & 1u)) & (az >> (o & 1u)) & 1u) != 0u;
+}
+
 template <bool EMIT>
-__device__ inline uint32_t node_groups(int i, const uint32_t* __restrict__ nstart,
+__device__ inline uint32_t node_groups(int i, const uint64_t* __restrict__ nprefix,
+                                       const uint32_t* __restrict__ nstart,
                                        const uint32_t* __restrict__ ncount,
                                        const uint32_t* __restrict__ nchild,
                                        const uint32_t* __restrict__ nfirst,
@@ -413,21 +424,36 @@ __device__ inline uint32_t node_groups(int i, const uint32_t* __restrict__ nstar
             }
         return k;
     }
-    uint32_t ng = 0, run_start = 0, run_count = 0;
     const uint32_t f = nfirst[i];
+    uint32_t dense = 0;  // octants of the children holding > G particles
+    for (uint32_t j = f; j < f + nc; ++j)
+        if (ncount[j] > kGroupMax)
+            dense |= 1u << static_cast<uint32_t>(nprefix[j] & 7u);
+    uint32_t ng = 0, run_start = 0, run_count = 0;
+    uint32_t bx = 0, by = 0, bz = 0;  // the run's octant box
     for (uint32_t j = f; j < f + nc; ++j) {
         const uint32_t cc = ncount[j];
         const bool small = cc <= kGroupMax;
-        if (run_count > 0 && (!small || run_count + cc > kGroupMax)) {
+        const uint32_t o = static_cast<uint32_t>(nprefix[j] & 7u);
+        const uint32_t nx = bx | (1u << ((o >> 2) & 1u)), ny = by | (1u << ((o >> 1) & 1u)),
+                       nz = bz | (1u << (o & 1u));
+        bool covers_dense = false;
+        for (uint32_t q = 0; q < 8; ++q)
+            covers_dense = covers_dense || (((dense >> q) & 1u) && octant_box_holds(nx, ny, nz, q));
+        if (run_count > 0 && (!small || run_count + cc > kGroupMax || covers_dense)) {
             if (EMIT)
                 groups[base + ng] = make_uint4(run_start, run_count, static_cast<uint32_t>(i), 0u);
             ++ng;
             run_count = 0;
+            bx = by = bz = 0;
         }
         if (small) {
             if (run_count == 0)
                 run_start = nstart[j];
             run_count += cc;
+            bx |= 1u << ((o >> 2) & 1u);
+            by |= 1u << ((o >> 1) & 1u);
+            bz |= 1u << (o & 1u);
         }
     }
     if (run_count > 0) {
@@ -439,24 +465,26 @@ __device__ inline uint32_t node_groups(int i, const uint32_t* __restrict__ nstar
 }
 
 __global__ __launch_bounds__(256) void group_count_kernel(
-    const uint32_t* __restrict__ nstart, const uint32_t* __restrict__ ncount,
-    const uint32_t* __restrict__ nchild, const uint32_t* __restrict__ nfirst,
-    const uint32_t* __restrict__ nparent, const int* __restrict__ info, uint32_t* __restrict__ gcnt)
+    const uint64_t* __restrict__ nprefix, const uint32_t* __restrict__ nstart,
+    const uint32_t* __restrict__ ncount, const uint32_t* __restrict__ nchild,
+    const uint32_t* __restrict__ nfirst, const uint32_t* __restrict__ nparent,
+    const int* __restrict__ info, uint32_t* __restrict__ gcnt)
 {
     const int total = info[kInfoTotal];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x)
-        gcnt[i] = node_groups<false>(i, nstart, ncount, nchild, nfirst, nparent, 0, nullptr);
+        gcnt[i] = node_groups<false>(i, nprefix, nstart, ncount, nchild, nfirst, nparent, 0, nullptr);
 }
 
 __global__ __launch_bounds__(256) void group_emit_kernel(
-    const uint32_t* __restrict__ nstart, const uint32_t* __restrict__ ncount,
-    const uint32_t* __restrict__ nchild, const uint32_t* __restrict__ nfirst,
-    const uint32_t* __restrict__ nparent, const uint32_t* __restrict__ goff,
+    const uint64_t* __restrict__ nprefix, const uint32_t* __restrict__ nstart,
+    const uint32_t* __restrict__ ncount, const uint32_t* __restrict__ nchild,
+    const uint32_t* __restrict__ nfirst, const uint32_t* __restrict__ nparent,
+    const uint32_t* __restrict__ goff,
     const int* __restrict__ info, uint4* __restrict__ groups)
 {
     const int total = info[kInfoTotal];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x)
-        node_groups<true>(i, nstart, ncount, nchild, nfirst, nparent, goff[i], groups);
+        node_groups<true>(i, nprefix, nstart, ncount, nchild, nfirst, nparent, goff[i], groups);
 }
 
 inline unsigned level_grid(size_t bound)
@@ -500,11 +528,11 @@ void launch_tree_build(Device& d, int leaf_depth, int bucket)
 void launch_groups(Device& d)
 {
     const unsigned g = level_grid(d.node_cap);
-    group_count_kernel<<<g, 256, 0, d.stream>>>(d.nstart, d.ncount, d.nchild, d.nfirst,
+    group_count_kernel<<<g, 256, 0, d.stream>>>(d.nprefix, d.nstart, d.ncount, d.nchild, d.nfirst,
                                                 d.nparent, d.level_info, d.gscan);
     launch_scan_exclusive(d.gscan, d.node_cap, d.level_info + kInfoTotal, 0, d.gscan_blocks,
                           nullptr, d.level_info + kInfoGroups, d.stream);
-    group_emit_kernel<<<g, 256, 0, d.stream>>>(d.nstart, d.ncount, d.nchild, d.nfirst,
+    group_emit_kernel<<<g, 256, 0, d.stream>>>(d.nprefix, d.nstart, d.ncount, d.nchild, d.nfirst,
                                                d.nparent, d.gscan, d.level_info, d.groups);
 }
 

@@ -88,16 +88,20 @@ __device__ inline float tpm_weight(float r, float inv_rs)
     return erfcf(a) + r * inv_rs * 0.5641895835477563f * __expf(-a * a);
 }
 
+// List evaluation. A lane takes entries j0, j0 + dj, ...: dj = 1 for groups
+// of more than 32 targets; smaller groups give each target dj lanes (see
+// walk_kernel), which split the list and are summed at the end of the group.
 template <int ORDER, bool TPM>
 __device__ inline void eval_cells(const float4* __restrict__ ca, const float4* __restrict__ cb,
-                                  const float2* __restrict__ cc, int ncell, float tx, float ty,
-                                  float tz, const WalkParams& P, float& ax, float& ay, float& az)
+                                  const float2* __restrict__ cc, int ncell, int j0, int dj,
+                                  float tx, float ty, float tz, const WalkParams& P, float& ax,
+                                  float& ay, float& az)
 {
     const float inv_rs = TPM ? 1.0f / P.rs : 0.0f;
     const float box = TPM ? P.box : 0.0f;
     const float ib = box > 0.0f ? 1.0f / box : 0.0f;
 #pragma unroll 2
-    for (int j = 0; j < ncell; ++j) {
+    for (int j = j0; j < ncell; j += dj) {
         const float4 a = ca[j];
         float dx = a.x - tx, dy = a.y - ty, dz = a.z - tz;
         if (TPM && box > 0.0f) {  // nearest image of this pair
@@ -105,7 +109,7 @@ __device__ inline void eval_cells(const float4* __restrict__ ca, const float4* _
             dy -= box * rintf(dy * ib);
             dz -= box * rintf(dz * ib);
         }
-        const float r2 = dx * dx + dy * dy + dz * dz + P.eps2;
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, P.eps2)));
         const float ri = __frsqrt_rn(r2);
         const float ri2 = ri * ri;
         const float ri3 = ri * ri2;
@@ -136,14 +140,32 @@ __device__ inline void eval_cells(const float4* __restrict__ ca, const float4* _
 }
 
 template <bool TPM>
-__device__ inline void eval_parts(const float4* __restrict__ pl, int np, float tx, float ty,
-                                  float tz, const WalkParams& P, float& ax, float& ay, float& az)
+__device__ inline void eval_parts(const float4* __restrict__ pl, int np, int j0, int dj, float tx,
+                                  float ty, float tz, const WalkParams& P, float& ax, float& ay,
+                                  float& az)
 {
+    if (!TPM && P.eps2 > 0.0f && P.min_d2 <= P.eps2) {
+        // every pair has r2 >= eps2 >= min_d2: the skip test can only drop a
+        // coincident pair, whose contribution is exactly zero anyway
+        const float eps2 = P.eps2;
+#pragma unroll 4
+        for (int j = j0; j < np; j += dj) {
+            const float4 a = pl[j];
+            const float dx = a.x - tx, dy = a.y - ty, dz = a.z - tz;
+            const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+            const float ri = __frsqrt_rn(r2);
+            const float s = a.w * ri * (ri * ri);
+            ax += s * dx;
+            ay += s * dy;
+            az += s * dz;
+        }
+        return;
+    }
     const float inv_rs = TPM ? 1.0f / P.rs : 0.0f;
     const float box = TPM ? P.box : 0.0f;
     const float ib = box > 0.0f ? 1.0f / box : 0.0f;
 #pragma unroll 4
-    for (int j = 0; j < np; ++j) {
+    for (int j = j0; j < np; j += dj) {
         const float4 a = pl[j];
         float dx = a.x - tx, dy = a.y - ty, dz = a.z - tz;
         if (TPM && box > 0.0f) {  // nearest image of this pair
@@ -243,6 +265,44 @@ __device__ inline void far_evaluate(const float* F, float dx, float dy, float dz
           0.5f * (F[11] * xx + F[16] * yy + F[18] * zz) + F[13] * xy + F[14] * xz + F[17] * yz;
 }
 
+// Far nodes of a group are queued in LDS (node indices) and folded into the
+// group's expansion in batches of up to kFarCap, with every lane taking whole
+// nodes; the lanes' partial sums are reduced once per batch into fsum. The
+// 19 accumulators are then live only inside the batch, not across the list
+// evaluation loops.
+constexpr int kFarCap = 256;
+
+__device__ inline void far_flush(const uint32_t* __restrict__ fl, int nfl,
+                                 const WalkNode* __restrict__ nodes, int rcx, int rcy, int rcz,
+                                 float ux, float uy, float uz, float gcx, float gcy, float gcz,
+                                 float eps2, float* __restrict__ fsum)
+{
+    float F[kFarTerms];
+#pragma unroll
+    for (int k = 0; k < kFarTerms; ++k)
+        F[k] = 0.0f;
+#pragma unroll 1
+    for (int i = static_cast<int>(threadIdx.x); i < nfl; i += 64) {
+        const WalkNode w = nodes[fl[i]];
+        const float rx = static_cast<float>(w.cx - rcx) * ux + w.comx - gcx;
+        const float ry = static_cast<float>(w.cy - rcy) * uy + w.comy - gcy;
+        const float rz = static_cast<float>(w.cz - rcz) * uz + w.comz - gcz;
+        far_accumulate(rx, ry, rz, w, eps2, F);
+    }
+#pragma unroll
+    for (int k = 0; k < kFarTerms; ++k) {
+        float v = F[k];
+        for (int o = 32; o > 0; o >>= 1)
+            v += __shfl_xor(v, o, 64);
+        F[k] = v;
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < kFarTerms; ++k)
+            fsum[k] += F[k];
+    }
+}
+
 template <int ORDER, int CRIT, bool TPM, bool REC, bool FAR>
 __global__ __launch_bounds__(64) void walk_kernel(
     WalkParams P, const WalkNode* __restrict__ nodes, const float4* __restrict__ spart,
@@ -262,6 +322,8 @@ __global__ __launch_bounds__(64) void walk_kernel(
     __shared__ uint32_t lfPre[64];
     __shared__ float lfSx[64], lfSy[64], lfSz[64];
     __shared__ int sGroup;
+    __shared__ uint32_t fl[FAR ? kFarCap : 1];
+    __shared__ float fsum[FAR ? kFarTerms : 1];
 
     const int lane = threadIdx.x;
     int* stk = stack_all + static_cast<size_t>(blockIdx.x) * static_cast<size_t>(stack_cap);
@@ -283,8 +345,14 @@ __global__ __launch_bounds__(64) void walk_kernel(
 
         const uint4 gr = groups[g];
         const int nact = static_cast<int>(gr.y);
-        const int s = static_cast<int>(gr.x) + lane;
-        const bool active = lane < nact;
+        // groups of <= 32 (<= 16) targets: 2 (4) lanes per target, each
+        // evaluating every 2nd (4th) list entry
+        const int dj = nact <= 16 ? 4 : (nact <= 32 ? 2 : 1);
+        const int tw = 64 / dj;  // lanes per list share
+        const int tl = lane & (tw - 1);
+        const int j0 = lane / tw;
+        const int s = static_cast<int>(gr.x) + tl;
+        const bool active = tl < nact;
         const int gnode = static_cast<int>(gr.z);
         const int rcx = nodes[gnode].cx, rcy = nodes[gnode].cy, rcz = nodes[gnode].cz;
 
@@ -314,11 +382,11 @@ __global__ __launch_bounds__(64) void walk_kernel(
         const float ghx = 0.5f * (gx1 - gx0), ghy = 0.5f * (gy1 - gy0), ghz = 0.5f * (gz1 - gz0);
 
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
-        float F[FAR ? kFarTerms : 1];
-        for (int k = 0; k < (FAR ? kFarTerms : 1); ++k)
-            F[k] = 0.0f;
         const float grad = FAR ? sqrtf(ghx * ghx + ghy * ghy + ghz * ghz) : 0.0f;
         unsigned long long nfar = 0;
+        int nfl = 0;
+        if (FAR && lane < kFarTerms)
+            fsum[lane] = 0.0f;
         int ncell = 0, npart = 0;
         int nrec = 0;
         int sp = 1;
@@ -438,20 +506,25 @@ __global__ __launch_bounds__(64) void walk_kernel(
                 nrec += static_cast<int>(__popcll(rm));
             }
 
-            if (FAR && accept) {
-                // far from the whole group: into this lane's share of the
-                // group's local expansion instead of the per-target list.
-                // The group radius weighs kFarGroupWeight times the node's:
-                // the expansion is second order in the offset from the group
-                // centre, with a larger error constant than the node's own
-                // second-order multipole.
-                const float rx = px - gcx, ry = py - gcy, rz = pz - gcz;
-                const float lim = w.bmax + kFarGroupWeight * grad;
-                if (lim * lim < P.far2 * (rx * rx + ry * ry + rz * rz)) {
-                    far_accumulate(rx, ry, rz, w, P.eps2, F);
-                    accept = false;
-                    ++nfar;
+            if (FAR) {
+                // far from the whole group: queued for the group's local
+                // expansion instead of the per-target list. The group radius
+                // weighs kFarGroupWeight times the node's: the expansion is
+                // second order in the offset from the group centre, with a
+                // larger error constant than the node's own second-order
+                // multipole.
+                bool far = false;
+                if (accept) {
+                    const float rx = px - gcx, ry = py - gcy, rz = pz - gcz;
+                    const float lim = w.bmax + kFarGroupWeight * grad;
+                    far = lim * lim < P.far2 * (rx * rx + ry * ry + rz * rz);
+                    accept = accept && !far;
                 }
+                const unsigned long long fm = __ballot(far);
+                if (far)
+                    fl[nfl + static_cast<int>(__popcll(fm & lt))] = static_cast<uint32_t>(node);
+                nfl += static_cast<int>(__popcll(fm));
+                nfar += __popcll(fm);
             }
 
             // 1) accepted nodes -> LDS cell list
@@ -496,8 +569,13 @@ __global__ __launch_bounds__(64) void walk_kernel(
             }
             __syncthreads();
 
+            if (FAR && nfl > kFarCap - 64) {
+                far_flush(fl, nfl, nodes, rcx, rcy, rcz, ux, uy, uz, gcx, gcy, gcz, P.eps2, fsum);
+                nfl = 0;
+            }
+
             if (ncell > kCellCap - 64) {
-                eval_cells<ORDER, TPM>(cA, cB, cC, ncell, tx, ty, tz, P, ax, ay, az);
+                eval_cells<ORDER, TPM>(cA, cB, cC, ncell, j0, dj, tx, ty, tz, P, ax, ay, az);
                 npc += static_cast<unsigned long long>(ncell);
                 ncell = 0;
                 __syncthreads();
@@ -525,28 +603,32 @@ __global__ __launch_bounds__(64) void walk_kernel(
                 npart += static_cast<int>(chunk);
                 done += chunk;
                 if (npart == kPartCap) {
-                    eval_parts<TPM>(pl, npart, tx, ty, tz, P, ax, ay, az);
+                    eval_parts<TPM>(pl, npart, j0, dj, tx, ty, tz, P, ax, ay, az);
                     npp += static_cast<unsigned long long>(npart);
                     npart = 0;
                     __syncthreads();
                 }
             }
         }
-        eval_cells<ORDER, TPM>(cA, cB, cC, ncell, tx, ty, tz, P, ax, ay, az);
-        eval_parts<TPM>(pl, npart, tx, ty, tz, P, ax, ay, az);
+        eval_cells<ORDER, TPM>(cA, cB, cC, ncell, j0, dj, tx, ty, tz, P, ax, ay, az);
+        eval_parts<TPM>(pl, npart, j0, dj, tx, ty, tz, P, ax, ay, az);
         npc += static_cast<unsigned long long>(ncell);
         npp += static_cast<unsigned long long>(npart);
+        for (int o = 32; o >= tw; o >>= 1) {  // the list shares of one target
+            ax += __shfl_xor(ax, o, 64);
+            ay += __shfl_xor(ay, o, 64);
+            az += __shfl_xor(az, o, 64);
+        }
         if (FAR) {
-            // sum the lanes' shares of the expansion, then evaluate it per target
-            for (int k = 0; k < kFarTerms; ++k) {
-                float v = F[k];
-                for (int o = 32; o > 0; o >>= 1)
-                    v += __shfl_xor(v, o, 64);
-                F[k] = v;
-            }
+            // the rest of the queue, then the group's expansion at each target
+            far_flush(fl, nfl, nodes, rcx, rcy, rcz, ux, uy, uz, gcx, gcy, gcz, P.eps2, fsum);
+            nfl = 0;
+            __syncthreads();
+            float F[kFarTerms];
+#pragma unroll
+            for (int k = 0; k < kFarTerms; ++k)
+                F[k] = fsum[k];
             far_evaluate(F, tx - gcx, ty - gcy, tz - gcz, ax, ay, az);
-            for (int o = 32; o > 0; o >>= 1)
-                nfar += __shfl_xor(nfar, o, 64);
         }
         // per-target interaction counts for the roofline (only active lanes are real targets)
         npp = npp * static_cast<unsigned long long>(nact);
@@ -565,7 +647,7 @@ __global__ __launch_bounds__(64) void walk_kernel(
             rec_n[g] = static_cast<uint32_t>(nrec < rec_cap ? nrec : rec_cap);
         __syncthreads();
 
-        if (active) {
+        if (active && j0 == 0) {
             // OctHostMath.cpp:50-57
             const float mag = sqrtf(ax * ax + ay * ay + az * az);
             if (P.max_acc > 0.0f && mag > P.max_acc && mag > 1.0e-6f) {

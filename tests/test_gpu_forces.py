@@ -246,3 +246,60 @@ def test_group_far_field_accuracy(theta, alpha, factor):
     # far_alpha trades accuracy for speed: bounded relative to the plain walk
     assert np.median(e1) < factor * np.median(e0) + 1e-5
     assert np.percentile(e1, 99) < 2 * factor * np.percentile(e0, 99) + 1e-4
+
+
+def _box_cluster(rng, n, centre, half):
+    return np.asarray(centre) + rng.uniform(-1.0, 1.0, (n, 3)) * np.asarray(half)
+
+
+@pytest.mark.parametrize("eps", [0.01, 0.8])
+def test_group_far_field_matches_local_expansion_oracle(eps):
+    """Two source clusters, each the only content of one root octant, and a
+    group of 40 light targets in a third: at theta 0.7 both source octants
+    are accepted whole, and at far_alpha 0.9 both enter the group's local
+    expansion. The walk must then equal the FP64 local expansion of
+    oracle/expansion.py (sum over the two nodes, about the target box
+    centre) to FP32 rounding, while the expansion itself differs from the
+    direct sum by its third-order truncation error (checked to be 30x the
+    tolerance, so a dropped or mis-signed term cannot pass)."""
+    import blitzar_amd as B
+    from oracle import expansion as X
+    say(f"case far field vs local expansion eps={eps}")
+    rng = np.random.default_rng(41)
+    tgt = _box_cluster(rng, 40, (-1.0, -1.0, -1.0), (0.4, 0.4, 0.4))
+    src_a = _box_cluster(rng, 100, (1.2, 1.2, 1.2), (0.3, 0.1, 0.15))
+    src_b = _box_cluster(rng, 80, (1.2, -1.0, 1.2), (0.1, 0.25, 0.1))
+    m_a = rng.uniform(0.5, 1.5, 100) * 0.01
+    m_b = rng.uniform(0.5, 1.5, 80) * 0.01
+    pos = np.vstack([tgt, src_a, src_b])
+    m = np.concatenate([np.full(40, 1e-15), m_a, m_b]).astype(np.float32)
+    with B.OctreeHip(len(pos), theta=0.7, softening=eps, multipole_order=2) as s:
+        s.set_timing(True)
+        s.upload(pos, None, m)
+        s.set_params(far_alpha=0.9)
+        acc = s.compute_accelerations()[:40].astype(np.float64)
+        tm = s.timings()
+    assert tm["far_interactions"] >= 2
+    # the FP64 expansion of both nodes about the centre of the target box
+    gc = 0.5 * (tgt.min(0) + tgt.max(0))
+    want = np.zeros((40, 3))
+    direct = np.zeros((40, 3))
+    for xs, ms in ((src_a, m_a), (src_b, m_b)):
+        ms = ms.astype(np.float32).astype(np.float64)
+        M = ms.sum()
+        com = (ms[:, None] * xs).sum(0) / M
+        y = xs - com
+        r2 = (y * y).sum(1)
+        S = np.einsum("n,ni,nj->ij", ms, y, y)
+        Q = 3.0 * S - r2 @ ms * np.eye(3)
+        q6 = (Q[0, 0], Q[0, 1], Q[0, 2], Q[1, 1], Q[1, 2], Q[2, 2])
+        A, J, H = X.local_expansion(M, com - gc, q6, float(r2 @ ms), eps)
+        want += X.evaluate(A, J, H, tgt - gc)
+        direct += X.direct(tgt, xs, ms, eps)
+    scale = np.linalg.norm(want, axis=1)
+    d_gpu = np.linalg.norm(acc - want, axis=1) / scale
+    d_trunc = np.linalg.norm(direct - want, axis=1) / scale
+    say(f"  GPU vs expansion max {d_gpu.max():.2e}; expansion vs direct max {d_trunc.max():.2e}")
+    tol = 2e-5
+    assert d_trunc.max() > 30 * tol
+    assert d_gpu.max() < tol
