@@ -42,7 +42,7 @@ class Params(C.Structure):
         ("periodic_box", C.c_float),
         ("pm_mesh", C.c_int32),
         ("far_alpha", C.c_float),
-        ("reserved", C.c_int32 * 1),
+        ("relative_alpha", C.c_float),
     ]
 
 

@@ -21,6 +21,7 @@
 namespace {
 
 constexpr int kDefaultWavesPerCu = 24;
+constexpr float kDefaultRelativeAlpha = 0.002f;
 constexpr size_t kStackCap = 16384;
 
 struct StepEvents {
@@ -35,6 +36,7 @@ struct bh_hip_handle {
     int leaf_depth = 1;
     int cu_count = 256;
     bool primed = false;      // acc holds a(x) of the current positions
+    bool amag_valid = false;  // amag holds |a| of a force evaluation of the current state
     bh::PmState pm;           // particle mesh of the periodic split (pm_mesh > 0)
     bool pm_failed = false;   // a rocFFT execution failed in the last call
     bool ad_primed = false;   // block time-step schedule initialised
@@ -111,7 +113,8 @@ bh_hip_params_t resolve(const bh_hip_params_t& in)
     p.theta = std::clamp(in.theta, min_theta, 4.0f);
     p.bucket_capacity = std::max(1, in.bucket_capacity);
     p.multipole_order = in.multipole_order >= 2 ? 2 : 1;
-    p.criterion = in.criterion == 1 || in.criterion == 2 ? in.criterion : 0;
+    p.criterion = in.criterion >= 1 && in.criterion <= 3 ? in.criterion : 0;
+    p.relative_alpha = in.relative_alpha > 0.0f ? in.relative_alpha : kDefaultRelativeAlpha;
     p.key_mode = in.key_mode == 1 ? 1 : 0;
     p.integrator = in.integrator == 1 ? 1 : 0;
     // CfgScenarioRuntime.cpp:63-68: max level <= 12, eta in [0.01, 1]
@@ -197,6 +200,8 @@ bool alloc_all(bh_hip_handle* h, uint64_t n)
             !dalloc(h, &d.acc[a], n, "acc"))
             return false;
     }
+    if (!dalloc(h, &d.amag, n, "amag") || !dalloc(h, &d.amag_b, n, "amag_b"))
+        return false;
     if (!dalloc(h, &d.mass, n, "mass") || !dalloc(h, &d.mass_b, n, "mass_b") ||
         !dalloc(h, &d.id, n, "id") || !dalloc(h, &d.id_b, n, "id_b") ||
         !dalloc(h, &d.rank, n, "rank") || !dalloc(h, &d.keys, n, "keys") ||
@@ -326,7 +331,33 @@ bh::WalkParams walk_params(const bh_hip_params_t& p)
     w.rcut2 = p.treepm_cutoff > 0.0f ? p.treepm_cutoff * p.treepm_cutoff : 0.0f;
     w.box = p.periodic_box;
     w.far2 = p.far_alpha > 0.0f ? p.far_alpha * p.far_alpha : 0.0f;
+    w.ralpha = p.relative_alpha;
     return w;
+}
+
+// A force walk. Under the relative criterion every walk records |a| per
+// target (the store permutation carries it), and the criterion reads it; the
+// first such walk on a new state is preceded by a COM walk at min(theta, 0.7)
+// over all groups. Other criteria neither read nor keep |a|.
+void launch_force_walk(bh_hip_handle* h, bh::WalkParams wp)
+{
+    if (wp.criterion != BH_HIP_CRITERION_RELATIVE) {
+        h->amag_valid = false;
+        h->d.amag_live = false;
+        bh::launch_walk(h->d, wp, h->stats);
+        return;
+    }
+    wp.amag = h->d.amag;
+    if (!h->amag_valid) {
+        bh::WalkParams w0 = wp;
+        w0.criterion = BH_HIP_CRITERION_COM;
+        w0.theta2 = std::min(wp.theta2, 0.49f);
+        w0.alevel = nullptr;
+        bh::launch_walk(h->d, w0, h->stats);
+    }
+    bh::launch_walk(h->d, wp, h->stats);
+    h->amag_valid = true;
+    h->d.amag_live = true;
 }
 
 bool lff_ready(bh_hip_handle* h) { return h->lff_cfg.enabled && h->lff.lists && h->lff.frag; }
@@ -360,12 +391,12 @@ void queue_force(bh_hip_handle* h)
     } else if (h->pm.N > 0) {
         bh::WalkParams wp = walk_params(h->p);
         wp.max_acc = 0.0f;  // clamped after the mesh force
-        bh::launch_walk(h->d, wp, h->stats);
+        launch_force_walk(h, wp);
         if (!bh::launch_pm_accel(h->d, h->pm, h->p.periodic_box, h->p.treepm_split_scale,
                                  h->p.max_acceleration))
             h->pm_failed = true;
     } else {
-        bh::launch_walk(h->d, walk_params(h->p), h->stats);
+        launch_force_walk(h, walk_params(h->p));
     }
     rec(ev ? ev->w1 : nullptr, h->d.stream);
     h->primed = true;
@@ -478,7 +509,7 @@ bh_hip_result_t step_adaptive(bh_hip_handle* h, double dt, int32_t nsteps)
         h->ad_tick = 0;
         StepEvents* ev = next_events(h);
         queue_build(h, ev);
-        bh::launch_walk(d, wp0, h->stats);
+        launch_force_walk(h, wp0);
         rec(ev ? ev->w1 : nullptr, d.stream);
         bh::launch_adaptive_init(d, a, 0u);
         h->ad_primed = true;
@@ -513,7 +544,7 @@ bh_hip_result_t step_adaptive(bh_hip_handle* h, double dt, int32_t nsteps)
             wp.alevel = d.alevel;
             wp.atick = tt;
             wp.alevels = a.levels;
-            bh::launch_walk(d, wp, h->stats);
+            launch_force_walk(h, wp);
             rec(ev ? ev->w1 : nullptr, d.stream);
             bh::launch_adaptive_correct(d, a, tt, h->stats + 2);
         }
@@ -558,6 +589,7 @@ void bh_hip_default_params(bh_hip_params_t* out)
     out->adaptive_max_level = 0;  // off (SrvSimulationServer.hpp:412-414: max level 4 when on)
     out->adaptive_eta = 0.25f;
     out->adaptive_flags = BH_HIP_ADAPTIVE_VELOCITY;
+    out->relative_alpha = kDefaultRelativeAlpha;
 }
 
 const char* bh_hip_version(void) { return "octree_hip 0.1 (gfx950)"; }
@@ -719,6 +751,7 @@ bh_hip_result_t bh_hip_upload(bh_hip_handle_t* h, const double* pos3, const doub
     h->primed = false;
     h->ad_primed = false;
     h->have_tree = false;
+    h->amag_valid = false;
     if (!ok)
         return BH_HIP_INTERNAL_ERROR;
     return finish(h, "upload");

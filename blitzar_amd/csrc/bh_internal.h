@@ -105,12 +105,16 @@ struct WalkParams {
     float eps2;
     float min_d2;
     float max_acc;      // <= 0: no clamp
-    int criterion;      // 0 = centre of mass, 1 = bounds
+    int criterion;      // 0 = centre of mass, 1 = bounds, 2 = extent, 3 = relative
     int order;          // 1 = monopole, 2 = quadrupole
     float rs;           // TreePM split scale (0 = Newton)
     float rcut2;        // TreePM cutoff squared (0 = none)
     float box = 0.0f;   // periodic cube side for the TreePM walk (0 = open)
     float far2 = 0.0f;  // group far field: (bmax + group radius)^2 < far2 * d^2 (0 = off)
+    float ralpha = 0.0f;  // relative criterion tolerance
+    // |a| per particle (store order): read by the relative criterion, written
+    // by every walk for the targets it evaluates (nullptr: neither)
+    float* amag = nullptr;
     // Block time steps: when alevel is set, only groups holding a particle
     // active at tick `atick` are walked (active: atick mod 2^(alevels -
     // level) == 0); the other groups keep their previous acc.
@@ -149,6 +153,9 @@ struct Device {
     uint32_t* id_b = nullptr;
     uint32_t* rank = nullptr;  // caller index -> store slot
     float* acc[3] = {nullptr, nullptr, nullptr};
+    float* amag = nullptr;    // |a| of the last force evaluation (relative criterion)
+    float* amag_b = nullptr;
+    bool amag_live = false;   // amag is in use: the store permutation carries it
 
     uint64_t* keys = nullptr;
     uint64_t* keys_b = nullptr;

@@ -137,6 +137,8 @@ bool load_config(const std::string& path, CaseConfig& c, std::ostream& err)
         {"octree_hip.pericentre", d(c.pericentre)},
         {"octree_hip.split_scale", f(c.treepm_split_scale)},
         {"octree_hip.cutoff", f(c.treepm_cutoff)},
+        {"octree_hip.far_alpha", f(c.far_alpha)},
+        {"octree_hip.relative_alpha", f(c.relative_alpha)},
         {"octree_hip.ic", s(c.preset)},
     };
     std::string line;
@@ -189,7 +191,8 @@ void write_config(std::ostream& o, const CaseConfig& c)
     o << "octree_hip(bucket=" << c.bucket << ", order=" << c.multipole_order
       << ", scale_radius=" << c.scale_radius << ", separation=" << c.separation
       << ", pericentre=" << c.pericentre << ", split_scale=" << c.treepm_split_scale
-      << ", cutoff=" << c.treepm_cutoff << ")\n";
+      << ", cutoff=" << c.treepm_cutoff << ", far_alpha=" << c.far_alpha
+      << ", relative_alpha=" << c.relative_alpha << ")\n";
 }
 
 std::string normalize_solver(const std::string& name)

@@ -57,6 +57,8 @@ struct CaseConfig {
     bool treepm_enabled = false;
     float treepm_split_scale = 0.0f;  // octree_hip(split_scale=...)
     float treepm_cutoff = 0.0f;       // octree_hip(cutoff=...)
+    float far_alpha = 0.0f;           // octree_hip(far_alpha=...): group far field
+    float relative_alpha = 0.0f;      // octree_hip(relative_alpha=...): 0 = solver default
     // octree_hip(...): this solver's own directive (ignored by the reference)
     uint32_t bucket = 16;
     int multipole_order = 2;

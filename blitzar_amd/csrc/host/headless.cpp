@@ -211,9 +211,10 @@ int main(int argc, char** argv)
     p.min_theta = cfg.min_theta;
     p.max_acceleration = cfg.max_acceleration;
     p.max_speed = cfg.sph_max_speed;
-    p.criterion = cfg.criterion == "bounds"   ? BH_HIP_CRITERION_BOUNDS
-                  : cfg.criterion == "extent" ? BH_HIP_CRITERION_EXTENT
-                                              : BH_HIP_CRITERION_COM;
+    p.criterion = cfg.criterion == "bounds"     ? BH_HIP_CRITERION_BOUNDS
+                  : cfg.criterion == "extent"   ? BH_HIP_CRITERION_EXTENT
+                  : cfg.criterion == "relative" ? BH_HIP_CRITERION_RELATIVE
+                                                : BH_HIP_CRITERION_COM;
     p.leaf_capacity = static_cast<int32_t>(cfg.leaf_capacity);
     p.bucket_capacity = static_cast<int32_t>(cfg.bucket);
     p.multipole_order = cfg.multipole_order;
@@ -221,6 +222,9 @@ int main(int argc, char** argv)
     p.integrator = euler ? BH_HIP_INTEGRATOR_EULER : BH_HIP_INTEGRATOR_LEAPFROG;
     p.treepm_split_scale = cfg.treepm_split_scale;
     p.treepm_cutoff = cfg.treepm_cutoff;
+    p.far_alpha = cfg.far_alpha;
+    if (cfg.relative_alpha > 0.0f)
+        p.relative_alpha = cfg.relative_alpha;
 
     char cerr_buf[512] = {0};
     bh_hip_handle_t* h = nullptr;

@@ -307,9 +307,10 @@ __global__ __launch_bounds__(256) void permute_kernel(
     const uint32_t* __restrict__ src, size_t n, const double* __restrict__ px,
     const double* __restrict__ py, const double* __restrict__ pz, const double* __restrict__ vx,
     const double* __restrict__ vy, const double* __restrict__ vz, const float* __restrict__ m,
-    const uint32_t* __restrict__ id, double* __restrict__ opx, double* __restrict__ opy,
-    double* __restrict__ opz, double* __restrict__ ovx, double* __restrict__ ovy,
-    double* __restrict__ ovz, float* __restrict__ om, uint32_t* __restrict__ oid)
+    const uint32_t* __restrict__ id, const float* __restrict__ am, double* __restrict__ opx,
+    double* __restrict__ opy, double* __restrict__ opz, double* __restrict__ ovx,
+    double* __restrict__ ovy, double* __restrict__ ovz, float* __restrict__ om,
+    uint32_t* __restrict__ oid, float* __restrict__ oam)
 {
     const size_t s = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (s >= n)
@@ -323,6 +324,8 @@ __global__ __launch_bounds__(256) void permute_kernel(
     ovz[s] = vz[j];
     om[s] = m[j];
     oid[s] = id[j];
+    if (am)
+        oam[s] = am[j];
 }
 
 // The block time-step arrays follow the store through the same permutation.
@@ -429,13 +432,16 @@ void launch_permute(Device& d)
     const unsigned blocks = static_cast<unsigned>((d.n + 255) / 256);
     permute_kernel<<<blocks, 256, 0, d.stream>>>(
         d.vals, d.n, d.pos[0], d.pos[1], d.pos[2], d.vel[0], d.vel[1], d.vel[2], d.mass, d.id,
-        d.pos_b[0], d.pos_b[1], d.pos_b[2], d.vel_b[0], d.vel_b[1], d.vel_b[2], d.mass_b, d.id_b);
+        d.amag_live ? d.amag : nullptr, d.pos_b[0], d.pos_b[1], d.pos_b[2], d.vel_b[0], d.vel_b[1],
+        d.vel_b[2], d.mass_b, d.id_b, d.amag_b);
     for (int a = 0; a < 3; ++a) {
         std::swap(d.pos[a], d.pos_b[a]);
         std::swap(d.vel[a], d.vel_b[a]);
     }
     std::swap(d.mass, d.mass_b);
     std::swap(d.id, d.id_b);
+    if (d.amag_live)
+        std::swap(d.amag, d.amag_b);
     if (d.alevel) {
         permute_adaptive_kernel<<<blocks, 256, 0, d.stream>>>(
             d.vals, d.n, d.aacc[0], d.aacc[1], d.aacc[2], d.alevel, d.atick, d.aacc_b[0],

@@ -120,16 +120,17 @@ __device__ inline void eval_cells(const float4* __restrict__ ca, const float4* _
             const float4 b = cb[j];
             const float2 c = cc[j];
             // b = (qxx, qxy, qxz, qyy), c = (qyz, eps^2 trs); qzz = -qxx - qyy
-            const float qx = b.x * dx + b.y * dy + b.z * dz;
-            const float qy = b.y * dx + b.w * dy + c.x * dz;
-            const float qz = b.z * dx + c.x * dy - (b.x + b.w) * dz;
-            const float dqd = dx * qx + dy * qy + dz * qz;
-            const float ri5 = ri3 * ri2;
-            const float s = (a.w * ri3 + 2.5f * (dqd - c.y) * ri5 * ri2) * wgt;
-            const float t = ri5 * wgt;
-            ax += s * dx - t * qx;
-            ay += s * dy - t * qy;
-            az += s * dz - t * qz;
+            const float qx = fmaf(b.z, dz, fmaf(b.y, dy, b.x * dx));
+            const float qy = fmaf(c.x, dz, fmaf(b.w, dy, b.y * dx));
+            const float qz = fmaf(-(b.x + b.w), dz, fmaf(c.x, dy, b.z * dx));
+            const float dqd = fmaf(dz, qz, fmaf(dy, qy, dx * qx));
+            // M/R^3 + 5/2 (dQd - eps^2 trs)/R^7 = (M + 5/2 (dQd - eps^2 trs)/R^4) / R^3
+            const float u = (dqd - c.y) * ri2;
+            const float s = fmaf(2.5f * u, ri2, a.w) * ri3 * wgt;
+            const float t = ri3 * ri2 * wgt;
+            ax = fmaf(-t, qx, fmaf(s, dx, ax));
+            ay = fmaf(-t, qy, fmaf(s, dy, ay));
+            az = fmaf(-t, qz, fmaf(s, dz, az));
         } else {
             const float s = a.w * ri3 * wgt;
             ax += s * dx;
@@ -380,6 +381,15 @@ __global__ __launch_bounds__(64) void walk_kernel(
         const float gz0 = wmin(active ? tz : 3.0e38f), gz1 = wmax(active ? tz : -3.0e38f);
         const float gcx = 0.5f * (gx0 + gx1), gcy = 0.5f * (gy0 + gy1), gcz = 0.5f * (gz0 + gz1);
         const float ghx = 0.5f * (gx1 - gx0), ghy = 0.5f * (gy1 - gy0), ghz = 0.5f * (gz1 - gz0);
+        // relative criterion: the group's smallest |a| of the last evaluation,
+        // floored at 1 % of its largest (a target whose force nearly cancels
+        // has no meaningful relative error and would open everything)
+        float rtol = 0.0f;
+        if (CRIT == 3) {
+            const float am = active ? P.amag[s] : 0.0f;
+            const float amin = wmin(active ? am : 3.0e38f), amax = wmax(am);
+            rtol = P.ralpha * fmaxf(amin, 0.01f * amax);
+        }
 
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
         const float grad = FAR ? sqrtf(ghx * ghx + ghy * ghy + ghz * ghz) : 0.0f;
@@ -462,6 +472,12 @@ __global__ __launch_bounds__(64) void walk_kernel(
                         ok = ok && b2 > 0.0f && size * size < P.theta2 * b2;
                     } else if (CRIT == 2) {
                         ok = ok && w.bmax * w.bmax < P.theta2 * (d2 + P.eps2);
+                    } else if (CRIT == 3) {
+                        const float r2 = d2 + P.eps2;
+                        const float b2 = w.bmax * w.bmax;
+                        const float err = ORDER >= 2 ? w.mass * b2 * w.bmax : w.mass * b2;
+                        const float lim = ORDER >= 2 ? rtol * r2 * r2 * sqrtf(r2) : rtol * r2 * r2;
+                        ok = ok && size * size < P.theta2 * r2 && err <= lim;
                     } else {
                         ok = ok && size * size < P.theta2 * (d2 + P.eps2);
                     }
@@ -659,6 +675,8 @@ __global__ __launch_bounds__(64) void walk_kernel(
             accx[s] = ax;
             accy[s] = ay;
             accz[s] = az;
+            if (P.amag)
+                P.amag[s] = sqrtf(ax * ax + ay * ay + az * az);
         }
     }
 }
@@ -706,11 +724,20 @@ void launch_walk(Device& d, const WalkParams& p, unsigned long long* stats)
             launch_walk_t<2, 1, false, false, true>(d, p, grid, stats);
         else if (p.criterion == 2)
             launch_walk_t<2, 2, false, false, true>(d, p, grid, stats);
+        else if (p.criterion == 3)
+            launch_walk_t<2, 3, false, false, true>(d, p, grid, stats);
         else
             launch_walk_t<2, 0, false, false, true>(d, p, grid, stats);
         return;
     }
-    if (p.criterion == 2) {
+    if (p.criterion == 3) {
+        if (p.order >= 2)
+            tpm ? launch_walk_t<2, 3, true>(d, p, grid, stats)
+                : launch_walk_t<2, 3, false>(d, p, grid, stats);
+        else
+            tpm ? launch_walk_t<1, 3, true>(d, p, grid, stats)
+                : launch_walk_t<1, 3, false>(d, p, grid, stats);
+    } else if (p.criterion == 2) {
         if (p.order >= 2)
             tpm ? launch_walk_t<2, 2, true>(d, p, grid, stats)
                 : launch_walk_t<2, 2, false>(d, p, grid, stats);

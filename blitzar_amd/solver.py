@@ -17,7 +17,7 @@ from . import _abi
 from ._abi import Params, Timings, lib
 
 SOLVER_NAMES = ("octree_hip", "octree-hip", "octree_gpu_hip")
-CRITERIA = {"com": 0, "center_of_mass": 0, "bounds": 1, "extent": 2}
+CRITERIA = {"com": 0, "center_of_mass": 0, "bounds": 1, "extent": 2, "relative": 3}
 INTEGRATORS = {"leapfrog": 0, "kdk": 0, "euler": 1}
 
 

@@ -42,8 +42,22 @@ typedef enum bh_hip_result {
 
 enum { BH_HIP_INTEGRATOR_LEAPFROG = 0, BH_HIP_INTEGRATOR_EULER = 1 };
 /* COM and BOUNDS are the reference's octree(criterion) values; EXTENT opens
- * by bmax / distance (bmax = radius about the COM enclosing the node). */
-enum { BH_HIP_CRITERION_COM = 0, BH_HIP_CRITERION_BOUNDS = 1, BH_HIP_CRITERION_EXTENT = 2 };
+ * by bmax / distance (bmax = radius about the COM enclosing the node).
+ * RELATIVE (no counterpart in the reference) accepts a node when its
+ * estimated truncation error is below relative_alpha times the group's
+ * smallest acceleration of the previous force evaluation:
+ *   quadrupole walk: M bmax^3 <= relative_alpha |a|min (d^2 + eps^2)^(5/2)
+ *   monopole walk:   M bmax^2 <= relative_alpha |a|min (d^2 + eps^2)^2
+ * (d: distance from the node's COM to the group's target box), and also
+ * passes the COM test at theta, which then bounds the opening angle. The
+ * first force evaluation after an upload primes |a| with a COM walk at
+ * min(theta, 0.7). */
+enum {
+    BH_HIP_CRITERION_COM = 0,
+    BH_HIP_CRITERION_BOUNDS = 1,
+    BH_HIP_CRITERION_EXTENT = 2,
+    BH_HIP_CRITERION_RELATIVE = 3
+};
 enum { BH_HIP_KEYS_MORTON63 = 0, BH_HIP_KEYS_REFERENCE30 = 1 };
 
 /*
@@ -93,7 +107,9 @@ typedef struct bh_hip_params {
      * per target, instead of being evaluated per target. 0 = off. Quadrupole
      * walk without TreePM only. */
     float far_alpha;
-    int32_t reserved[1];
+    /* Error tolerance of BH_HIP_CRITERION_RELATIVE (> 0; ignored by the other
+     * criteria). */
+    float relative_alpha;
 } bh_hip_params_t;
 
 /* adaptive_flags: also bound the step by eta * eps / |v| (the reference does) */

@@ -303,3 +303,43 @@ def test_group_far_field_matches_local_expansion_oracle(eps):
     tol = 2e-5
     assert d_trunc.max() > 30 * tol
     assert d_gpu.max() < tol
+
+
+def test_relative_criterion_accuracy_and_priming():
+    """The relative criterion (a node's estimated error below alpha times the
+    group's smallest previous |a|): the first evaluation after an upload is
+    primed by a COM walk, a tighter alpha gives a smaller error, and the
+    error stays bounded after KDK steps that permute the store (the |a| of
+    the last evaluation follows the permutation)."""
+    import blitzar_amd as B
+    n = 60000
+    say(f"case relative criterion n={n}")
+    pos, vel, m = B.plummer(n, 7)
+    t = np.arange(0, n, 29, dtype=np.uint32)
+    med = {}
+    with B.OctreeHip(n, theta=1.1, softening=0.005, criterion="relative",
+                     relative_alpha=0.004, integrator="leapfrog") as s:
+        s.upload(pos, vel, m)
+        ref = s.direct_forces_fp64(t)
+        first = s.compute_accelerations()  # COM priming walk + relative walk
+        assert np.all(np.isfinite(first))
+        for alpha in (0.016, 0.004, 0.001):
+            s.set_params(relative_alpha=alpha)
+            e = rel(s.compute_accelerations()[t], ref)
+            med[alpha] = float(np.median(e))
+            say(f"  alpha {alpha}: median {med[alpha]:.2e} p99 {np.percentile(e, 99):.2e}")
+        assert med[0.001] < med[0.004] < med[0.016]
+        assert med[0.004] < 1e-3
+        s.step(1e-3, 3)
+        p3, _ = s.download()
+        s.set_params(relative_alpha=0.004)
+        a3 = s.compute_accelerations()
+        r3 = s.direct_forces_fp64(t)
+    e3 = rel(a3[t], r3)
+    say(f"  after 3 steps: median {np.median(e3):.2e}")
+    assert np.median(e3) < 1.5 * med[0.004] + 1e-5
+    with B.OctreeHip(n, theta=1.1, softening=0.005, criterion="relative",
+                     relative_alpha=0.004) as s2:
+        s2.upload(p3, None, m)
+        b3 = s2.compute_accelerations()  # fresh state: primed again
+    assert np.median(rel(b3[t], r3)) < 1.5 * med[0.004] + 1e-5
