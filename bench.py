@@ -12,15 +12,17 @@ One step = drift/kick + tree rebuild (AABB, keys, radix sort, tree, moments)
 + walk + kick, all on the device with the state resident in HBM. Synthetic
 initial conditions (generated on the host, uploaded once).
 
-The opening angle is the accuracy knob of the metric: the headline runs at
-theta = 0.75 with the group far-field expansion (far_alpha 0.75: accepted
-nodes far from a whole target group enter one local expansion per group);
-on this state that holds the median error near 7.5e-4, a quarter below the
-bar (theta 0.8 reaches 8.9e-4 for 1 % less walk time), and the same run
-also times the plain walk at theta = 0.5, the
-opening angle the workload is usually quoted at, under "theta_0.5". Both
-force errors are measured in the run. `--lff` adds the learned far-field
-correction (theta 0.9, trained on a second realisation, seed 12).
+The opening test is the accuracy knob of the metric. The headline runs the
+relative criterion: a node is accepted when its estimated truncation error,
+M bmax^3 / (d^2 + eps^2)^(5/2), is below alpha = 0.02 times the group's
+smallest |a| of the previous force evaluation, with the opening angle bounded
+by theta = 2. On this state that holds the median error near 7.3e-4, about a
+quarter below the bar. The same run also times the plain COM walk at
+theta = 0.5 under "theta_0.5", the opening angle the workload is usually
+quoted at. Both force errors are measured in the run. `--criterion com
+--theta 0.75 --far-alpha 0.75` runs the geometric criterion with the group
+far-field expansion instead; `--lff` adds the learned far-field correction
+(theta 0.9, trained on a second realisation, seed 12).
 
 Prints one JSON line (see the keys below). `--gpus N` with N > 1 exits: this
 build runs on one GPU.
@@ -99,15 +101,16 @@ def cpu_baseline(pos, m, theta, eps, n_targets, seed=123):
     }
 
 
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "walk_build_merger8M_theta0.75_far0.75_pmc.md")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "walk_build_merger8M_relative0.02_theta2_pmc.md")
 
 
 def walk_traffic(args, n):
     """HBM bytes per walk launch from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this workload (2 x FETCH + WRITE, calibrated on the
     streaming kick/drift kernel), or None for other workloads."""
-    if not (args.workload == "merger" and n == 8388608 and args.theta == 0.75 and args.order == 2
-            and args.far_alpha == 0.75):
+    if not (args.workload == "merger" and n == 8388608 and args.order == 2 and args.bucket == 16
+            and args.criterion == "relative" and args.theta == 2.0
+            and args.relative_alpha == 0.02 and args.far_alpha == 0.0):
         return None
     try:
         with open(PMC_SUMMARY) as f:
@@ -117,6 +120,13 @@ def walk_traffic(args, n):
     except OSError:
         pass
     return None
+
+
+def workload_name(args, n):
+    crit = (f"relative{args.relative_alpha}" if args.criterion == "relative" else args.criterion)
+    return (f"{'hernquist_merger' if args.workload == 'merger' else 'plummer'}_{n}_{crit}"
+            f"_theta{args.theta}_{'quadrupole' if args.order >= 2 else 'monopole'}"
+            f"{f'_far{args.far_alpha}' if args.far_alpha > 0 else ''}")
 
 
 def force_error(acc, ref, samples):
@@ -335,9 +345,12 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=8388608)
     ap.add_argument("--workload", default="merger", choices=["merger", "plummer"])
-    ap.add_argument("--theta", type=float, default=0.75)
-    ap.add_argument("--far-alpha", type=float, default=0.75,
-                    help="group far-field expansion (0 = plain walk)")
+    ap.add_argument("--criterion", default="relative", choices=["relative", "com", "bounds", "extent"])
+    ap.add_argument("--theta", type=float, default=2.0,
+                    help="opening angle (under the relative criterion: its upper bound)")
+    ap.add_argument("--relative-alpha", type=float, default=0.02)
+    ap.add_argument("--far-alpha", type=float, default=0.0,
+                    help="group far-field expansion (0 = off)")
     ap.add_argument("--ref-theta", type=float, default=0.5)
     ap.add_argument("--ref-steps", type=int, default=5)
     ap.add_argument("--lff", action="store_true")
@@ -370,7 +383,8 @@ def main(argv=None):
     log(f"[bench] state {args.workload} n={n} generated in {time.perf_counter() - t0:.2f} s")
 
     s = B.OctreeHip(n, theta=args.theta, softening=args.eps, multipole_order=args.order,
-                    bucket_capacity=args.bucket, integrator="leapfrog", far_alpha=args.far_alpha)
+                    bucket_capacity=args.bucket, integrator="leapfrog", far_alpha=args.far_alpha,
+                    criterion=args.criterion, relative_alpha=args.relative_alpha)
     s.upload(pos, vel, m)
     s.set_timing(True)
     s.compute_accelerations(to_host=False)
@@ -404,7 +418,7 @@ def main(argv=None):
         pos = p_now
 
     # the same state at the reference opening angle (timed the same way)
-    s.set_params(theta=args.ref_theta, far_alpha=0.0)
+    s.set_params(theta=args.ref_theta, far_alpha=0.0, criterion="com")
     s.step(args.dt, 1)
     t0 = time.perf_counter()
     s.step(args.dt, args.ref_steps)
@@ -462,10 +476,10 @@ def main(argv=None):
         "dtype": "fp32",
         "data": "synthetic",
         "config": {
-            "workload": (f"{'hernquist_merger' if args.workload == 'merger' else 'plummer'}"
-                         f"_{n}_theta{args.theta}_{'quadrupole' if args.order >= 2 else 'monopole'}"
-                         f"{f'_far{args.far_alpha}' if args.far_alpha > 0 else ''}"),
-            "n": n, "theta": args.theta, "far_alpha": args.far_alpha, "softening": args.eps,
+            "workload": workload_name(args, n),
+            "n": n, "criterion": args.criterion, "theta": args.theta,
+            "relative_alpha": args.relative_alpha if args.criterion == "relative" else None,
+            "far_alpha": args.far_alpha, "softening": args.eps,
             "dt": args.dt,
             "integrator": "leapfrog_kdk_fp64", "bucket": args.bucket,
             "parallelism": "none (1 GPU)",
@@ -479,7 +493,7 @@ def main(argv=None):
             "kernel": "walk_kernel", "bound": "mfma", "pipe": "fp32 VALU",
             "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS, "traffic": walk_traffic(args, n),
-            "traffic_source": "profiles/walk_build_merger8M_theta0.75_far0.75_pmc.md (rocprofv3 PMC)",
+            "traffic_source": f"profiles/{os.path.basename(PMC_SUMMARY)} (rocprofv3 PMC)",
             "interactions_per_launch": {"pp": tm["pp_interactions"] / walks,
                                         "pc": tm["pc_interactions"] / walks,
                                         "far_node_group": tm["far_interactions"] / walks,
