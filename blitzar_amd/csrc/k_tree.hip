@@ -74,33 +74,55 @@ __global__ void tree_root_kernel(uint64_t* nprefix, uint8_t* nlevel, uint32_t* n
 }
 
 // Boundaries of the 8 child octants of a node spanning sorted keys [a, b):
-// bnd[o] = first index whose level-(L+1) digit is >= o; bnd[8] = b.
+// bnd[o] = first index whose level-(L+1) digit is >= o; bnd[8] = b. Keys are
+// sorted, so bnd[o] = a + #{keys in [a, b) with digit < o}: nodes of up to 16
+// keys count with independent loads; larger nodes run the seven binary
+// searches interleaved (seven independent loads per step, one dependent step
+// per halving instead of seven).
 __device__ inline void child_bounds(const unsigned long long* __restrict__ keys, uint32_t a,
                                     uint32_t b, int shift, uint32_t* bnd)
 {
     bnd[0] = a;
     bnd[8] = b;
     if (b - a <= 16) {
-        uint32_t i = a;
-        for (int o = 1; o < 8; ++o) {
-            while (i < b && static_cast<int>((keys[i] >> shift) & 7u) < o)
-                ++i;
-            bnd[o] = i;
+        uint32_t below[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (uint32_t k = 0; k < 16; ++k) {
+            if (a + k < b) {
+                const uint32_t dg = static_cast<uint32_t>((keys[a + k] >> shift) & 7u);
+#pragma unroll
+                for (uint32_t o = 1; o < 8; ++o)
+                    below[o] += dg < o ? 1u : 0u;
+            }
         }
+#pragma unroll
+        for (int o = 1; o < 8; ++o)
+            bnd[o] = a + below[o];
         return;
     }
-    uint32_t lo_prev = a;
+    uint32_t lo[8], hi[8];
+#pragma unroll
     for (int o = 1; o < 8; ++o) {
-        uint32_t lo = lo_prev, hi = b;
-        while (lo < hi) {
-            const uint32_t mid = lo + ((hi - lo) >> 1);
-            if (static_cast<int>((keys[mid] >> shift) & 7u) < o)
-                lo = mid + 1;
-            else
-                hi = mid;
+        lo[o] = a;
+        hi[o] = b;
+    }
+    for (uint32_t len = b - a; len > 0; len >>= 1) {  // ceil(log2(b - a + 1)) rounds
+#pragma unroll
+        for (int o = 1; o < 8; ++o) {
+            if (lo[o] < hi[o]) {
+                const uint32_t mid = lo[o] + ((hi[o] - lo[o]) >> 1);
+                if (static_cast<int>((keys[mid] >> shift) & 7u) < o)
+                    lo[o] = mid + 1;
+                else
+                    hi[o] = mid;
+            }
         }
-        bnd[o] = lo;
-        lo_prev = lo;
+    }
+#pragma unroll
+    for (int o = 1; o < 8; ++o) {
+        // a search still open after the rounds cannot happen: each round at
+        // least halves hi - lo, and b - a halvings reach zero
+        bnd[o] = lo[o];
     }
 }
 
@@ -282,6 +304,7 @@ __global__ __launch_bounds__(256) void tree_moments_kernel(
             const double oy = geom->lo_d[1] + static_cast<double>(c[1]) * geom->unit_d[1];
             const double oz = geom->lo_d[2] + static_cast<double>(c[2]) * geom->unit_d[2];
             float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+#pragma unroll 4
             for (uint32_t s = first; s < first + count; ++s) {
                 const float x = static_cast<float>(px[s] - ox);
                 const float y = static_cast<float>(py[s] - oy);
@@ -299,6 +322,7 @@ __global__ __launch_bounds__(256) void tree_moments_kernel(
                 cy = sy / M;
                 cz = sz / M;
             }
+#pragma unroll 4
             for (uint32_t s = first; s < first + count; ++s) {
                 const float4 p = spart[s];
                 const float dx = p.x - cx, dy = p.y - cy, dz = p.z - cz;
@@ -513,9 +537,9 @@ void launch_tree_build(Device& d, int leaf_depth, int bucket)
     for (int L = 0; L < kMaxLevel; ++L) {
         const size_t bound = level_bound(L, d.n, d.node_cap);
         const unsigned g = level_grid(bound * 64);  // a wavefront per node on small levels
-        tree_count_kernel<<<g, 256, 0, d.stream>>>(keys, d.nstart, d.ncount, d.nchild, d.nfirst,
-                                                   d.scan_tmp, d.bnd_tmp, d.level_info, L,
-                                                   leaf_depth, static_cast<uint32_t>(bucket));
+        tree_count_kernel<<<g, 256, 0, d.stream>>>(
+            keys, d.nstart, d.ncount, d.nchild, d.nfirst, d.scan_tmp, d.bnd_tmp, d.level_info, L,
+            leaf_depth, static_cast<uint32_t>(bucket));
         launch_scan_exclusive(d.scan_tmp, bound, d.level_info + kLevels + L, 0, d.scan_blocks,
                               nullptr, d.level_info + kLevels + L + 1, d.stream);
         tree_level_fix_kernel<<<1, 64, 0, d.stream>>>(d.level_info, L, cap, d.err);
