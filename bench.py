@@ -14,10 +14,11 @@ initial conditions (generated on the host, uploaded once).
 
 The opening test is the accuracy knob of the metric. The headline runs the
 relative criterion: a node is accepted when its estimated truncation error,
-M bmax^3 / (d^2 + eps^2)^(5/2), is below alpha = 0.02 times the group's
+M bmax^3 / (d^2 + eps^2)^(5/2), is below alpha = 0.015 times the group's
 smallest |a| of the previous force evaluation, with the opening angle bounded
-by theta = 2. On this state that holds the median error near 7.3e-4, about a
-quarter below the bar. The same run also times the plain COM walk at
+by theta = 2; accepted nodes whose quadrupole term, M bmax^2 / (d^2 +
+eps^2)^2, is below 0.1 alpha |a| enter as point masses. On this state that
+holds the median error near 6.6e-4, a third below the bar. The same run also times the plain COM walk at
 theta = 0.5 under "theta_0.5", the opening angle the workload is usually
 quoted at. Both force errors are measured in the run. `--criterion com
 --theta 0.75 --far-alpha 0.75` runs the geometric criterion with the group
@@ -101,7 +102,7 @@ def cpu_baseline(pos, m, theta, eps, n_targets, seed=123):
     }
 
 
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "walk_build_merger8M_relative0.02_theta2_pmc.md")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "walk_build_merger8M_relative0.015_mono0.1_pmc.md")
 
 
 def walk_traffic(args, n):
@@ -110,7 +111,8 @@ def walk_traffic(args, n):
     streaming kick/drift kernel), or None for other workloads."""
     if not (args.workload == "merger" and n == 8388608 and args.order == 2 and args.bucket == 16
             and args.criterion == "relative" and args.theta == 2.0
-            and args.relative_alpha == 0.02 and args.far_alpha == 0.0):
+            and args.relative_alpha == 0.015 and args.relative_mono == 0.1
+            and args.far_alpha == 0.0):
         return None
     try:
         with open(PMC_SUMMARY) as f:
@@ -123,7 +125,8 @@ def walk_traffic(args, n):
 
 
 def workload_name(args, n):
-    crit = (f"relative{args.relative_alpha}" if args.criterion == "relative" else args.criterion)
+    crit = (f"relative{args.relative_alpha}_mono{args.relative_mono}" if args.criterion == "relative"
+            else args.criterion)
     return (f"{'hernquist_merger' if args.workload == 'merger' else 'plummer'}_{n}_{crit}"
             f"_theta{args.theta}_{'quadrupole' if args.order >= 2 else 'monopole'}"
             f"{f'_far{args.far_alpha}' if args.far_alpha > 0 else ''}")
@@ -348,7 +351,9 @@ def main(argv=None):
     ap.add_argument("--criterion", default="relative", choices=["relative", "com", "bounds", "extent"])
     ap.add_argument("--theta", type=float, default=2.0,
                     help="opening angle (under the relative criterion: its upper bound)")
-    ap.add_argument("--relative-alpha", type=float, default=0.02)
+    ap.add_argument("--relative-alpha", type=float, default=0.015)
+    ap.add_argument("--relative-mono", type=float, default=0.1,
+                    help="relative criterion: point-mass tolerance as a fraction of alpha (0 = off)")
     ap.add_argument("--far-alpha", type=float, default=0.0,
                     help="group far-field expansion (0 = off)")
     ap.add_argument("--ref-theta", type=float, default=0.5)
@@ -384,7 +389,8 @@ def main(argv=None):
 
     s = B.OctreeHip(n, theta=args.theta, softening=args.eps, multipole_order=args.order,
                     bucket_capacity=args.bucket, integrator="leapfrog", far_alpha=args.far_alpha,
-                    criterion=args.criterion, relative_alpha=args.relative_alpha)
+                    criterion=args.criterion, relative_alpha=args.relative_alpha,
+                    relative_mono=args.relative_mono)
     s.upload(pos, vel, m)
     s.set_timing(True)
     s.compute_accelerations(to_host=False)
@@ -479,6 +485,7 @@ def main(argv=None):
             "workload": workload_name(args, n),
             "n": n, "criterion": args.criterion, "theta": args.theta,
             "relative_alpha": args.relative_alpha if args.criterion == "relative" else None,
+            "relative_mono": args.relative_mono if args.criterion == "relative" else None,
             "far_alpha": args.far_alpha, "softening": args.eps,
             "dt": args.dt,
             "integrator": "leapfrog_kdk_fp64", "bucket": args.bucket,
@@ -495,6 +502,7 @@ def main(argv=None):
             "frac": achieved / PEAK_FP32_TFLOPS, "traffic": walk_traffic(args, n),
             "traffic_source": f"profiles/{os.path.basename(PMC_SUMMARY)} (rocprofv3 PMC)",
             "interactions_per_launch": {"pp": tm["pp_interactions"] / walks,
+                                        "pp_point_mass_cells": tm["mono_interactions"] / walks,
                                         "pc": tm["pc_interactions"] / walks,
                                         "far_node_group": tm["far_interactions"] / walks,
                                         "far_evaluations": tm["far_evaluations"] / walks},

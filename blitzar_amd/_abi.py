@@ -43,6 +43,7 @@ class Params(C.Structure):
         ("pm_mesh", C.c_int32),
         ("far_alpha", C.c_float),
         ("relative_alpha", C.c_float),
+        ("relative_mono", C.c_float),
     ]
 
 
@@ -62,6 +63,7 @@ class Timings(C.Structure):
         ("particle_steps", C.c_uint64),
         ("far_interactions", C.c_uint64),
         ("far_evaluations", C.c_uint64),
+        ("mono_interactions", C.c_uint64),
     ]
 
 

@@ -115,6 +115,7 @@ bh_hip_params_t resolve(const bh_hip_params_t& in)
     p.multipole_order = in.multipole_order >= 2 ? 2 : 1;
     p.criterion = in.criterion >= 1 && in.criterion <= 3 ? in.criterion : 0;
     p.relative_alpha = in.relative_alpha > 0.0f ? in.relative_alpha : kDefaultRelativeAlpha;
+    p.relative_mono = std::max(0.0f, in.relative_mono);
     p.key_mode = in.key_mode == 1 ? 1 : 0;
     p.integrator = in.integrator == 1 ? 1 : 0;
     // CfgScenarioRuntime.cpp:63-68: max level <= 12, eta in [0.01, 1]
@@ -216,7 +217,7 @@ bool alloc_all(bh_hip_handle* h, uint64_t n)
     if (!dalloc(h, &d.level_info, 2 * bh::kLevels + 8, "level info") ||
         !dalloc(h, &d.geom, 1, "geometry") || !dalloc(h, &d.aabb_bits, 8, "aabb") ||
         !dalloc(h, &d.err, 4, "error flags") || !dalloc(h, &d.work_counter, 4, "work counter") ||
-        !dalloc(h, &h->stats, 5, "walk stats"))
+        !dalloc(h, &h->stats, 6, "walk stats"))
         return false;
     d.diag_cap = 6 * 1024 + 8 * 256 + 16;  // AABB partials, energy partials
     if (!dalloc(h, &d.diag, d.diag_cap, "diagnostics"))
@@ -332,6 +333,7 @@ bh::WalkParams walk_params(const bh_hip_params_t& p)
     w.box = p.periodic_box;
     w.far2 = p.far_alpha > 0.0f ? p.far_alpha * p.far_alpha : 0.0f;
     w.ralpha = p.relative_alpha;
+    w.rmono = p.relative_mono;
     return w;
 }
 
@@ -407,7 +409,7 @@ void begin_call(bh_hip_handle* h)
 {
     h->events_used = 0;
     h->integrate_ms = 0.0;
-    bh::launch_fill_u32(reinterpret_cast<uint32_t*>(h->stats), 0u, 10, h->d.stream);
+    bh::launch_fill_u32(reinterpret_cast<uint32_t*>(h->stats), 0u, 12, h->d.stream);
 }
 
 // After a synchronised call: fold event times and counters into h->t.
@@ -427,13 +429,14 @@ void end_call(bh_hip_handle* h)
         }
         t.integrate_ms = h->integrate_ms;
     }
-    unsigned long long st[5] = {0, 0, 0, 0, 0};
+    unsigned long long st[6] = {0, 0, 0, 0, 0, 0};
     if (hipMemcpy(st, h->stats, sizeof(st), hipMemcpyDeviceToHost) == hipSuccess) {
         t.pp_interactions = st[0];
         t.pc_interactions = st[1];
         t.particle_steps = st[2];
         t.far_interactions = st[3];
         t.far_evaluations = st[4];
+        t.mono_interactions = st[5];
     }
     int info[2 * bh::kLevels + 4];
     if (hipMemcpy(info, h->d.level_info, sizeof(info), hipMemcpyDeviceToHost) == hipSuccess) {

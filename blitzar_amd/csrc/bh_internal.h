@@ -112,6 +112,7 @@ struct WalkParams {
     float box = 0.0f;   // periodic cube side for the TreePM walk (0 = open)
     float far2 = 0.0f;  // group far field: (bmax + group radius)^2 < far2 * d^2 (0 = off)
     float ralpha = 0.0f;  // relative criterion tolerance
+    float rmono = 0.0f;   // relative criterion: point-mass tolerance as a fraction of ralpha
     // |a| per particle (store order): read by the relative criterion, written
     // by every walk for the targets it evaluates (nullptr: neither)
     float* amag = nullptr;

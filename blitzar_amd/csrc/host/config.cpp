@@ -139,6 +139,7 @@ bool load_config(const std::string& path, CaseConfig& c, std::ostream& err)
         {"octree_hip.cutoff", f(c.treepm_cutoff)},
         {"octree_hip.far_alpha", f(c.far_alpha)},
         {"octree_hip.relative_alpha", f(c.relative_alpha)},
+        {"octree_hip.relative_mono", f(c.relative_mono)},
         {"octree_hip.ic", s(c.preset)},
     };
     std::string line;
@@ -192,7 +193,8 @@ void write_config(std::ostream& o, const CaseConfig& c)
       << ", scale_radius=" << c.scale_radius << ", separation=" << c.separation
       << ", pericentre=" << c.pericentre << ", split_scale=" << c.treepm_split_scale
       << ", cutoff=" << c.treepm_cutoff << ", far_alpha=" << c.far_alpha
-      << ", relative_alpha=" << c.relative_alpha << ")\n";
+      << ", relative_alpha=" << c.relative_alpha << ", relative_mono=" << c.relative_mono
+      << ")\n";
 }
 
 std::string normalize_solver(const std::string& name)

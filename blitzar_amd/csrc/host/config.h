@@ -59,6 +59,7 @@ struct CaseConfig {
     float treepm_cutoff = 0.0f;       // octree_hip(cutoff=...)
     float far_alpha = 0.0f;           // octree_hip(far_alpha=...): group far field
     float relative_alpha = 0.0f;      // octree_hip(relative_alpha=...): 0 = solver default
+    float relative_mono = 0.0f;       // octree_hip(relative_mono=...): point-mass cells, 0 = off
     // octree_hip(...): this solver's own directive (ignored by the reference)
     uint32_t bucket = 16;
     int multipole_order = 2;

@@ -225,6 +225,7 @@ int main(int argc, char** argv)
     p.far_alpha = cfg.far_alpha;
     if (cfg.relative_alpha > 0.0f)
         p.relative_alpha = cfg.relative_alpha;
+    p.relative_mono = cfg.relative_mono;
 
     char cerr_buf[512] = {0};
     bh_hip_handle_t* h = nullptr;

@@ -384,12 +384,14 @@ __global__ __launch_bounds__(64) void walk_kernel(
         // relative criterion: the group's smallest |a| of the last evaluation,
         // floored at 1 % of its largest (a target whose force nearly cancels
         // has no meaningful relative error and would open everything)
-        float rtol = 0.0f;
+        float rtol = 0.0f, rtol_mono = 0.0f;
         if (CRIT == 3) {
             const float am = active ? P.amag[s] : 0.0f;
             const float amin = wmin(active ? am : 3.0e38f), amax = wmax(am);
             rtol = P.ralpha * fmaxf(amin, 0.01f * amax);
+            rtol_mono = P.rmono * rtol;
         }
+        unsigned long long nmono = 0;
 
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
         const float grad = FAR ? sqrtf(ghx * ghx + ghy * ghy + ghz * ghz) : 0.0f;
@@ -423,7 +425,7 @@ __global__ __launch_bounds__(64) void walk_kernel(
             }
             __syncthreads();
 
-            bool accept = false, direct = false, open = false;
+            bool accept = false, direct = false, open = false, mono = false;
             WalkNode w = {};
             float sx = 0.0f, sy = 0.0f, sz = 0.0f;
             float px = 0.0f, py = 0.0f, pz = 0.0f;
@@ -478,6 +480,8 @@ __global__ __launch_bounds__(64) void walk_kernel(
                         const float err = ORDER >= 2 ? w.mass * b2 * w.bmax : w.mass * b2;
                         const float lim = ORDER >= 2 ? rtol * r2 * r2 * sqrtf(r2) : rtol * r2 * r2;
                         ok = ok && size * size < P.theta2 * r2 && err <= lim;
+                        // quadrupole term within tolerance too: a point mass suffices
+                        mono = ORDER >= 2 && ok && w.mass * b2 <= rtol_mono * r2 * r2;
                     } else {
                         ok = ok && size * size < P.theta2 * (d2 + P.eps2);
                     }
@@ -543,7 +547,26 @@ __global__ __launch_bounds__(64) void walk_kernel(
                 nfar += __popcll(fm);
             }
 
-            // 1) accepted nodes -> LDS cell list
+            // 1) accepted nodes -> LDS cell list (point masses -> particle list)
+            if (CRIT == 3) {
+                mono = mono && accept;
+                accept = accept && !mono;
+                const unsigned long long mm = __ballot(mono);
+                const int nm = static_cast<int>(__popcll(mm));
+                if (nm > 0) {
+                    if (npart + nm >= kPartCap) {  // uniform; keeps npart < kPartCap
+                        eval_parts<TPM>(pl, npart, j0, dj, tx, ty, tz, P, ax, ay, az);
+                        npp += static_cast<unsigned long long>(npart);
+                        npart = 0;
+                        __syncthreads();
+                    }
+                    if (mono)
+                        pl[npart + static_cast<int>(__popcll(mm & lt))] =
+                            make_float4(px, py, pz, w.mass);
+                    npart += nm;
+                    nmono += static_cast<unsigned long long>(nm);
+                }
+            }
             const unsigned long long am = __ballot(accept);
             if (accept) {
                 const int slot = ncell + static_cast<int>(__popcll(am & lt));
@@ -652,6 +675,8 @@ __global__ __launch_bounds__(64) void walk_kernel(
         if (stats && lane == 0) {
             atomicAdd(&stats[0], npp);
             atomicAdd(&stats[1], npc);
+            if (CRIT == 3 && nmono)
+                atomicAdd(&stats[5], nmono * static_cast<unsigned long long>(nact));
             if (FAR) {
                 atomicAdd(&stats[3], nfar);                                        // node-group
                 atomicAdd(&stats[4], static_cast<unsigned long long>(nact));      // evaluations

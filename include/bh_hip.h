@@ -110,6 +110,11 @@ typedef struct bh_hip_params {
     /* Error tolerance of BH_HIP_CRITERION_RELATIVE (> 0; ignored by the other
      * criteria). */
     float relative_alpha;
+    /* BH_HIP_CRITERION_RELATIVE with the quadrupole: an accepted node whose
+     * quadrupole term is also small, M bmax^2 <= relative_mono *
+     * relative_alpha |a|min (d^2 + eps^2)^2, is evaluated as a point mass
+     * (softened monopole) instead of a second-order cell. 0 = off. */
+    float relative_mono;
 } bh_hip_params_t;
 
 /* adaptive_flags: also bound the step by eta * eps / |v| (the reference does) */
@@ -135,6 +140,8 @@ typedef struct bh_hip_timings {
     uint64_t particle_steps;  /* block time steps: particles corrected (sum over slices) */
     uint64_t far_interactions;  /* group far field: node-group expansions (sum over walks) */
     uint64_t far_evaluations;   /* group far field: per-target expansion evaluations */
+    uint64_t mono_interactions; /* relative_mono: accepted nodes evaluated as point masses
+                                   (per target; also counted in pp_interactions) */
 } bh_hip_timings_t;
 
 /* Fill `out` with the defaults (reference config defaults, CfgConfig.hpp). */

@@ -75,8 +75,8 @@ def test_create_rejects_invalid_periodic_settings_without_device(fields, why):
 def test_params_layout_matches_header():
     from blitzar_amd import _abi
     # 16 four-byte configuration fields + 6 block-step / periodic / mesh / far fields
-    # + relative_alpha
-    assert C.sizeof(_abi.Params) == 4 * 23
+    # + relative_alpha + relative_mono
+    assert C.sizeof(_abi.Params) == 4 * 24
     assert C.sizeof(_abi.LffConfig) == 20
 
 

@@ -343,3 +343,34 @@ def test_relative_criterion_accuracy_and_priming():
         s2.upload(p3, None, m)
         b3 = s2.compute_accelerations()  # fresh state: primed again
     assert np.median(rel(b3[t], r3)) < 1.5 * med[0.004] + 1e-5
+
+
+def test_relative_point_mass_cells():
+    """relative_mono: accepted nodes whose quadrupole term is within
+    relative_mono * alpha |a| are evaluated as softened point masses. A small
+    fraction keeps the error of the second-order walk; the point-mass count
+    grows with the fraction; 0 leaves only nodes with bmax = 0 (single
+    particles, exact as point masses)."""
+    import blitzar_amd as B
+    n = 60000
+    say(f"case relative point masses n={n}")
+    pos, vel, m = B.plummer(n, 7)
+    t = np.arange(0, n, 29, dtype=np.uint32)
+    out = {}
+    with B.OctreeHip(n, theta=2.0, softening=0.005, criterion="relative",
+                     relative_alpha=0.004) as s:
+        s.set_timing(True)
+        s.upload(pos, vel, m)
+        ref = s.direct_forces_fp64(t)
+        s.compute_accelerations()
+        for k in (0.0, 0.1, 1.0):
+            s.set_params(relative_mono=k)
+            a = s.compute_accelerations()
+            tm = s.timings()
+            out[k] = (float(np.median(rel(a[t], ref))), tm["mono_interactions"],
+                      tm["pc_interactions"])
+            say(f"  mono {k}: median {out[k][0]:.2e} point masses {out[k][1]} cells {out[k][2]}")
+    assert out[0.1][1] > out[0.0][1] and out[1.0][1] > out[0.1][1]
+    assert out[1.0][2] < out[0.1][2] < out[0.0][2]
+    assert out[0.1][0] < 1.3 * out[0.0][0] + 1e-5
+    assert out[1.0][0] < 1e-2
