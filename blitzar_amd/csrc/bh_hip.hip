@@ -174,6 +174,7 @@ bool alloc_nodes(bh_hip_handle* h, size_t cap)
     dfree(h, d.wnode);
     dfree(h, d.scan_tmp);
     dfree(h, d.bnd_tmp);
+    dfree(h, d.big_list);
     dfree(h, d.scan_blocks);
     dfree(h, d.nparent);
     dfree(h, d.gscan);
@@ -186,6 +187,7 @@ bool alloc_nodes(bh_hip_handle* h, size_t cap)
            dalloc(h, &d.nfirst, cap, "node first") && dalloc(h, &d.nlevel, cap, "node level") &&
            dalloc(h, &d.wnode, cap, "walk nodes") && dalloc(h, &d.scan_tmp, scan_n, "scan tmp") &&
            dalloc(h, &d.bnd_tmp, 8 * scan_n, "child bounds") &&
+           dalloc(h, &d.big_list, scan_n, "large-node list") &&
            dalloc(h, &d.scan_blocks, bh::scan_block_sums_words(scan_n), "scan blocks") &&
            dalloc(h, &d.nparent, cap, "node parent") && dalloc(h, &d.gscan, cap, "group scan") &&
            dalloc(h, &d.gscan_blocks, bh::scan_block_sums_words(cap), "group scan blocks");

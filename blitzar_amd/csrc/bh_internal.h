@@ -16,6 +16,7 @@ constexpr uint32_t kGroupMax = 64;      // targets per walk group (one wavefront
 constexpr int kInfoTotal = 2 * kLevels;      // level_info slot: total nodes
 constexpr int kInfoMaxLevel = 2 * kLevels + 1;  // deepest populated level
 constexpr int kInfoGroups = 2 * kLevels + 2;    // number of walk groups
+constexpr int kInfoBig = 2 * kLevels + 3;       // large nodes of the current level (tree build)
 
 // Learned far-field MLP: 16 inputs (15 features + pad) -> 64 -> 64 -> 3.
 constexpr int kLffInputs = 16;
@@ -183,6 +184,7 @@ struct Device {
 
     uint32_t* scan_tmp = nullptr;  // per-level scan scratch
     uint32_t* bnd_tmp = nullptr;   // per-level child bounds (8 per node) from count to emit
+    uint32_t* big_list = nullptr;  // per-level indices of nodes searched by a whole wavefront
     uint32_t* scan_blocks = nullptr;
     int* level_info = nullptr;  // [0..22) base, [22..44) count, [44] total, [45] max level, [46] groups
 

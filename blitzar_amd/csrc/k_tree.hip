@@ -70,6 +70,7 @@ __global__ void tree_root_kernel(uint64_t* nprefix, uint8_t* nlevel, uint32_t* n
         info[kLevels] = 1;  // count[0]
         info[2 * kLevels] = 1;
         info[2 * kLevels + 1] = 0;
+        info[kInfoBig] = 0;
     }
 }
 
@@ -167,44 +168,92 @@ __device__ inline void wave_child_bounds(const unsigned long long* __restrict__ 
         bnd[q] = __shfl(lo, 8 * q, 64);  // lane 8q holds boundary q
 }
 
+// Nodes of more than this many keys are searched by a whole wavefront even
+// on levels run thread-per-node: a dense clump can hold 10^5..10^6 keys in
+// one node, and one thread's 20 dependent search rounds would set the time
+// of the whole level. tree_count_kernel lists them; tree_count_big_kernel
+// spreads them over wavefronts (the ninefold bracketing takes about 6 rounds).
+constexpr uint32_t kWaveSearchMin = 2048;
+
+__device__ inline void store_count(uint32_t* __restrict__ nchild, uint32_t* __restrict__ nfirst,
+                                   uint32_t* __restrict__ scan_tmp, uint32_t* __restrict__ bnd_tmp,
+                                   int node, int k, uint32_t a, bool split, const uint32_t* bnd)
+{
+    uint32_t nc = 0;
+    if (split) {
+        for (int o = 0; o < 8; ++o)
+            nc += bnd[o + 1] > bnd[o] ? 1u : 0u;
+        // bounds 1..8 for the emit pass (bound 0 is the node start)
+        uint4* out = reinterpret_cast<uint4*>(bnd_tmp + static_cast<size_t>(k) * 8);
+        out[0] = make_uint4(bnd[1], bnd[2], bnd[3], bnd[4]);
+        out[1] = make_uint4(bnd[5], bnd[6], bnd[7], bnd[8]);
+    }
+    nchild[node] = nc;
+    nfirst[node] = a;  // leaves keep their particle start; parents are overwritten
+    scan_tmp[k] = nc;
+}
+
 __global__ __launch_bounds__(256) void tree_count_kernel(
     const unsigned long long* __restrict__ keys, const uint32_t* __restrict__ nstart,
     const uint32_t* __restrict__ ncount, uint32_t* __restrict__ nchild,
     uint32_t* __restrict__ nfirst, uint32_t* __restrict__ scan_tmp,
-    uint32_t* __restrict__ bnd_tmp, int* __restrict__ info, int level, int leaf_depth,
-    uint32_t bucket)
+    uint32_t* __restrict__ bnd_tmp, uint32_t* __restrict__ big_list, int* __restrict__ info,
+    int level, int leaf_depth, uint32_t bucket)
 {
     const int base = level_base(info)[level];
     const int cnt = level_count(info)[level];
     const int shift = 3 * (kMaxLevel - level - 1);
-    const bool wave_mode = cnt <= kWavePerNodeMax;  // uniform across the grid
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int stride = wave_mode ? (gridDim.x * blockDim.x) >> 6 : gridDim.x * blockDim.x;
-    for (int k = wave_mode ? tid >> 6 : tid; k < cnt; k += stride) {
+    if (cnt <= kWavePerNodeMax) {  // uniform across the grid: a wavefront per node
+        const int nwaves = static_cast<int>((gridDim.x * blockDim.x) >> 6);
+        for (int k = tid >> 6; k < cnt; k += nwaves) {
+            const int node = base + k;
+            const uint32_t a = nstart[node], c = ncount[node];
+            const bool split = level < kMaxLevel && (level < leaf_depth || c > bucket);
+            uint32_t bnd[9];
+            if (split)  // uniform across the wave
+                wave_child_bounds(keys, a, c, shift, bnd);
+            if ((threadIdx.x & 63) == 0)
+                store_count(nchild, nfirst, scan_tmp, bnd_tmp, node, k, a, split, bnd);
+        }
+        return;
+    }
+    for (int k = tid; k < cnt; k += gridDim.x * blockDim.x) {
         const int node = base + k;
         const uint32_t a = nstart[node], c = ncount[node];
         const bool split = level < kMaxLevel && (level < leaf_depth || c > bucket);
-        uint32_t nc = 0;
-        if (split) {  // uniform across the wave in wave mode
-            uint32_t bnd[9];
-            if (wave_mode)
-                wave_child_bounds(keys, a, c, shift, bnd);
-            else
-                child_bounds(keys, a, a + c, shift, bnd);
-            for (int o = 0; o < 8; ++o)
-                nc += bnd[o + 1] > bnd[o] ? 1u : 0u;
-            if (!wave_mode || (threadIdx.x & 63) == 0) {
-                // bounds 1..8 for the emit pass (bound 0 is the node start)
-                uint4* out = reinterpret_cast<uint4*>(bnd_tmp + static_cast<size_t>(k) * 8);
-                out[0] = make_uint4(bnd[1], bnd[2], bnd[3], bnd[4]);
-                out[1] = make_uint4(bnd[5], bnd[6], bnd[7], bnd[8]);
-            }
+        if (split && c > kWaveSearchMin) {
+            big_list[atomicAdd(&info[kInfoBig], 1)] = static_cast<uint32_t>(k);
+            continue;
         }
-        if (!wave_mode || (threadIdx.x & 63) == 0) {
-            nchild[node] = nc;
-            nfirst[node] = a;  // leaves keep their particle start; parents are overwritten
-            scan_tmp[k] = nc;
-        }
+        uint32_t bnd[9];
+        if (split)
+            child_bounds(keys, a, a + c, shift, bnd);
+        store_count(nchild, nfirst, scan_tmp, bnd_tmp, node, k, a, split, bnd);
+    }
+}
+
+// The large nodes tree_count_kernel listed: a wavefront per node.
+__global__ __launch_bounds__(256) void tree_count_big_kernel(
+    const unsigned long long* __restrict__ keys, const uint32_t* __restrict__ nstart,
+    const uint32_t* __restrict__ ncount, uint32_t* __restrict__ nchild,
+    uint32_t* __restrict__ nfirst, uint32_t* __restrict__ scan_tmp,
+    uint32_t* __restrict__ bnd_tmp, const uint32_t* __restrict__ big_list,
+    const int* __restrict__ info, int level)
+{
+    const int base = info[level];  // level_base
+    const int nbig = info[kInfoBig];
+    const int shift = 3 * (kMaxLevel - level - 1);
+    const int nwaves = static_cast<int>((gridDim.x * blockDim.x) >> 6);
+    for (int i = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6); i < nbig;
+         i += nwaves) {
+        const int k = static_cast<int>(big_list[i]);
+        const int node = base + k;
+        const uint32_t a = nstart[node], c = ncount[node];
+        uint32_t bnd[9];
+        wave_child_bounds(keys, a, c, shift, bnd);
+        if ((threadIdx.x & 63) == 0)
+            store_count(nchild, nfirst, scan_tmp, bnd_tmp, node, k, a, true, bnd);
     }
 }
 
@@ -229,6 +278,7 @@ __global__ void tree_level_fix_kernel(int* info, int level, int node_cap, int* e
     if (c > 0)
         info[2 * kLevels + 1] = level + 1;  // deepest populated level
     info[2 * kLevels] = nb + c;             // total nodes
+    info[kInfoBig] = 0;                     // large-node list of the next level
 }
 
 // Children of the nodes of `level` from the bounds tree_count_kernel left in
@@ -538,8 +588,12 @@ void launch_tree_build(Device& d, int leaf_depth, int bucket)
         const size_t bound = level_bound(L, d.n, d.node_cap);
         const unsigned g = level_grid(bound * 64);  // a wavefront per node on small levels
         tree_count_kernel<<<g, 256, 0, d.stream>>>(
-            keys, d.nstart, d.ncount, d.nchild, d.nfirst, d.scan_tmp, d.bnd_tmp, d.level_info, L,
-            leaf_depth, static_cast<uint32_t>(bucket));
+            keys, d.nstart, d.ncount, d.nchild, d.nfirst, d.scan_tmp, d.bnd_tmp, d.big_list,
+            d.level_info, L, leaf_depth, static_cast<uint32_t>(bucket));
+        if (bound > kWavePerNodeMax)  // levels that can run thread-per-node
+            tree_count_big_kernel<<<64, 256, 0, d.stream>>>(keys, d.nstart, d.ncount, d.nchild,
+                                                            d.nfirst, d.scan_tmp, d.bnd_tmp,
+                                                            d.big_list, d.level_info, L);
         launch_scan_exclusive(d.scan_tmp, bound, d.level_info + kLevels + L, 0, d.scan_blocks,
                               nullptr, d.level_info + kLevels + L + 1, d.stream);
         tree_level_fix_kernel<<<1, 64, 0, d.stream>>>(d.level_info, L, cap, d.err);

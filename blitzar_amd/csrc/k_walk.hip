@@ -91,12 +91,13 @@ __device__ inline float tpm_weight(float r, float inv_rs)
 // List evaluation. A lane takes entries j0, j0 + dj, ...: dj = 1 for groups
 // of more than 32 targets; smaller groups give each target dj lanes (see
 // walk_kernel), which split the list and are summed at the end of the group.
-template <int ORDER, bool TPM>
-__device__ inline void eval_cells(const float4* __restrict__ ca, const float4* __restrict__ cb,
-                                  const float2* __restrict__ cc, int ncell, int j0, int dj,
-                                  float tx, float ty, float tz, const WalkParams& P, float& ax,
-                                  float& ay, float& az)
+template <int ORDER, bool TPM, int DJ>
+__device__ inline void eval_cells_dj(const float4* __restrict__ ca, const float4* __restrict__ cb,
+                                     const float2* __restrict__ cc, int ncell, int j0, float tx,
+                                     float ty, float tz, const WalkParams& P, float& ax, float& ay,
+                                     float& az)
 {
+    constexpr int dj = DJ;  // a compile-time stride keeps the unrolled LDS reads on one base
     const float inv_rs = TPM ? 1.0f / P.rs : 0.0f;
     const float box = TPM ? P.box : 0.0f;
     const float ib = box > 0.0f ? 1.0f / box : 0.0f;
@@ -140,11 +141,12 @@ __device__ inline void eval_cells(const float4* __restrict__ ca, const float4* _
     }
 }
 
-template <bool TPM>
-__device__ inline void eval_parts(const float4* __restrict__ pl, int np, int j0, int dj, float tx,
-                                  float ty, float tz, const WalkParams& P, float& ax, float& ay,
-                                  float& az)
+template <bool TPM, int DJ>
+__device__ inline void eval_parts_dj(const float4* __restrict__ pl, int np, int j0, float tx,
+                                     float ty, float tz, const WalkParams& P, float& ax, float& ay,
+                                     float& az)
 {
+    constexpr int dj = DJ;
     if (!TPM && P.eps2 > 0.0f && P.min_d2 <= P.eps2) {
         // every pair has r2 >= eps2 >= min_d2: the skip test can only drop a
         // coincident pair, whose contribution is exactly zero anyway
@@ -188,6 +190,33 @@ __device__ inline void eval_parts(const float4* __restrict__ pl, int np, int j0,
         ay += s * dy;
         az += s * dz;
     }
+}
+
+template <int ORDER, bool TPM>
+__device__ inline void eval_cells(const float4* __restrict__ ca, const float4* __restrict__ cb,
+                                  const float2* __restrict__ cc, int ncell, int j0, int dj,
+                                  float tx, float ty, float tz, const WalkParams& P, float& ax,
+                                  float& ay, float& az)
+{
+    if (dj == 1)
+        eval_cells_dj<ORDER, TPM, 1>(ca, cb, cc, ncell, j0, tx, ty, tz, P, ax, ay, az);
+    else if (dj == 2)
+        eval_cells_dj<ORDER, TPM, 2>(ca, cb, cc, ncell, j0, tx, ty, tz, P, ax, ay, az);
+    else
+        eval_cells_dj<ORDER, TPM, 4>(ca, cb, cc, ncell, j0, tx, ty, tz, P, ax, ay, az);
+}
+
+template <bool TPM>
+__device__ inline void eval_parts(const float4* __restrict__ pl, int np, int j0, int dj, float tx,
+                                  float ty, float tz, const WalkParams& P, float& ax, float& ay,
+                                  float& az)
+{
+    if (dj == 1)
+        eval_parts_dj<TPM, 1>(pl, np, j0, tx, ty, tz, P, ax, ay, az);
+    else if (dj == 2)
+        eval_parts_dj<TPM, 2>(pl, np, j0, tx, ty, tz, P, ax, ay, az);
+    else
+        eval_parts_dj<TPM, 4>(pl, np, j0, tx, ty, tz, P, ax, ay, az);
 }
 
 // Group far field: the local (Taylor) expansion about the group centre of an
@@ -305,7 +334,12 @@ __device__ inline void far_flush(const uint32_t* __restrict__ fl, int nfl,
 }
 
 template <int ORDER, int CRIT, bool TPM, bool REC, bool FAR>
-__global__ __launch_bounds__(64) void walk_kernel(
+#ifndef BH_WALK_MIN_WAVES
+#define BH_WALK_MIN_WAVES 5
+#endif
+// Walks without the far field are held to 96 VGPRs (5 waves per SIMD): the
+// traversal is latency-bound and needs the resident waves.
+__global__ __launch_bounds__(64, FAR ? 1 : BH_WALK_MIN_WAVES) void walk_kernel(
     WalkParams P, const WalkNode* __restrict__ nodes, const float4* __restrict__ spart,
     const uint32_t* __restrict__ sleaf, const RootGeom* __restrict__ geom,
     const uint4* __restrict__ groups, const int* __restrict__ info,

@@ -165,3 +165,32 @@ def test_sort_pairs_edge_cases():
     low = k & np.uint64(2 ** 30 - 1)
     o = np.argsort(low, kind="stable")
     assert np.array_equal(vo, o.astype(np.uint32))
+
+
+def test_topology_with_a_dense_clump(oracle):
+    """A level run thread-per-node (more than 4,096 nodes) that also holds
+    nodes of more than 2,048 keys: those are listed and searched by whole
+    wavefronts in a second kernel. 160,000 uniform particles give the level
+    past the leaf depth 32,768 nodes; a clump of 240,000 particles in a
+    1e-4-wide cube puts nodes of up to 240,000 keys on the same levels."""
+    rng = np.random.default_rng(77)
+    n_u, n_c = 160000, 240000
+    pos = np.vstack([rng.random((n_u, 3)), 0.3 + 1e-4 * rng.random((n_c, 3))])
+    n = len(pos)
+    m = np.full(n, 1.0 / n, np.float32)
+    say(f"case topology dense clump n={n}")
+    with _solver(n, leaf_capacity=16, bucket_capacity=16) as s:
+        s.upload(pos, None, m)
+        s.build_tree()
+        t = s.dump_tree()
+    sk, _ = oracle.stable_sort(oracle.keys(pos, 0))
+    d = oracle.leaf_depth(n, 16)
+    ref = oracle.hip_tree(sk, d, 16)
+    lv, cnt = ref["level"], ref["count"]
+    per_level = np.bincount(lv)
+    big_levels = [L for L in range(len(per_level))
+                  if per_level[L] > 4096 and np.any(cnt[lv == L] > 2048)]
+    assert big_levels  # the case exercises the large-node list
+    assert len(t["level"]) == len(ref["level"])
+    for key in ("level", "prefix", "start", "count", "first_child", "nchild"):
+        assert np.array_equal(t[key].astype(np.int64), ref[key].astype(np.int64)), key
