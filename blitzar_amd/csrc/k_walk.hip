@@ -373,10 +373,14 @@ __global__ __launch_bounds__(64, FAR ? 1 : BH_WALK_MIN_WAVES) void walk_kernel(
         if (lane == 0)
             sGroup = static_cast<int>(atomicAdd(work, 1u));
         __syncthreads();
-        const int g = sGroup;
+        const int ticket = sGroup;
         __syncthreads();
-        if (g >= ngroups)
+        if (ticket >= ngroups)
             break;
+        // Groups are stored in the breadth-first order of their nodes, shallow
+        // (sparse, cheap) levels first; taking them deepest first puts the
+        // expensive dense groups at the start instead of in the tail.
+        const int g = ngroups - 1 - ticket;
 
         const uint4 gr = groups[g];
         const int nact = static_cast<int>(gr.y);
