@@ -243,6 +243,9 @@ __global__ __launch_bounds__(256) void potential_kernel(
 }
 
 inline unsigned blocks_for(size_t n) { return static_cast<unsigned>((n + 255) / 256); }
+// grids of the kernels that reduce into one device counter: at most 2,048
+// blocks (8 per CU), each adding once
+inline unsigned reduce_blocks(size_t n) { return std::min(blocks_for(n), 2048u); }
 
 }  // namespace
 
@@ -391,29 +394,15 @@ __global__ __launch_bounds__(256) void adaptive_drift_kernel(
     vz[i] = z;
 }
 
-// Corrector for the particles active at `tick` (their new force is in acc).
-__global__ __launch_bounds__(256) void adaptive_correct_kernel(
+// Corrector of one particle active at `tick` (its new force is in acc).
+__device__ inline void correct_one(
     double* __restrict__ px, double* __restrict__ py, double* __restrict__ pz,
     double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
     const float* __restrict__ nx, const float* __restrict__ ny, const float* __restrict__ nz,
     float* __restrict__ bx, float* __restrict__ by, float* __restrict__ bz,
-    uint32_t* __restrict__ level, uint32_t* __restrict__ last, size_t n, AdaptiveParams a,
-    uint32_t tick, unsigned long long* __restrict__ count)
+    uint32_t* __restrict__ level, uint32_t* __restrict__ last, size_t i, const AdaptiveParams& a,
+    uint32_t tick)
 {
-    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    bool due = false;
-    if (i < n) {
-        const uint32_t lv = min(level[i], static_cast<uint32_t>(a.levels));
-        const uint32_t cadence = 1u << (static_cast<uint32_t>(a.levels) - lv);
-        due = (tick & (cadence - 1u)) == 0u;
-    }
-    if (count) {
-        const unsigned long long m = __ballot(due);
-        if ((threadIdx.x & 63) == 0 && m)
-            atomicAdd(count, static_cast<unsigned long long>(__popcll(m)));
-    }
-    if (!due)
-        return;
     const double h = static_cast<double>(tick - last[i]) * a.quantum;
     const double cx = static_cast<double>(nx[i]) - bx[i];
     const double cy = static_cast<double>(ny[i]) - by[i];
@@ -434,23 +423,69 @@ __global__ __launch_bounds__(256) void adaptive_correct_kernel(
     level[i] = adaptive_level(nx[i], ny[i], nz[i], x, y, z, a);
 }
 
+// Corrector for the particles active at `tick` (their new force is in acc).
+__global__ __launch_bounds__(256) void adaptive_correct_kernel(
+    double* __restrict__ px, double* __restrict__ py, double* __restrict__ pz,
+    double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
+    const float* __restrict__ nx, const float* __restrict__ ny, const float* __restrict__ nz,
+    float* __restrict__ bx, float* __restrict__ by, float* __restrict__ bz,
+    uint32_t* __restrict__ level, uint32_t* __restrict__ last, size_t n, AdaptiveParams a,
+    uint32_t tick, unsigned long long* __restrict__ count)
+{
+    // Grid-stride over the particles; the count of corrected particles is
+    // summed per block and added with one atomic per block (a single counter
+    // every wave updates would serialise them).
+    __shared__ unsigned long long block_count[4];
+    unsigned long long ndue = 0;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const uint32_t lv = min(level[i], static_cast<uint32_t>(a.levels));
+        const uint32_t cadence = 1u << (static_cast<uint32_t>(a.levels) - lv);
+        if ((tick & (cadence - 1u)) != 0u)
+            continue;
+        ++ndue;
+        correct_one(px, py, pz, vx, vy, vz, nx, ny, nz, bx, by, bz, level, last, i, a, tick);
+    }
+    if (count) {
+        for (int o = 32; o > 0; o >>= 1)
+            ndue += __shfl_xor(ndue, o, 64);
+        if ((threadIdx.x & 63) == 0)
+            block_count[threadIdx.x >> 6] = ndue;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned long long t =
+                block_count[0] + block_count[1] + block_count[2] + block_count[3];
+            if (t)
+                atomicAdd(count, t);
+        }
+    }
+}
+
 // Smallest tick after t at which some particle is due: min over particles of
 // (t / cadence + 1) * cadence (ticks count from a multiple of 2^L).
 __global__ __launch_bounds__(256) void adaptive_next_due_kernel(const uint32_t* __restrict__ level,
                                                                 size_t n, int L, uint32_t t,
                                                                 uint32_t* __restrict__ out)
 {
-    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    // grid-stride; one atomicMin per block
+    __shared__ uint32_t block_due[4];
     uint32_t due = 0xffffffffu;
-    if (i < n) {
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
         const uint32_t lv = min(level[i], static_cast<uint32_t>(L));
         const uint32_t c = 1u << (static_cast<uint32_t>(L) - lv);
-        due = (t / c + 1u) * c;
+        due = min(due, (t / c + 1u) * c);
     }
     for (int o = 32; o > 0; o >>= 1)
         due = min(due, static_cast<uint32_t>(__shfl_xor(static_cast<int>(due), o, 64)));
-    if ((threadIdx.x & 63) == 0 && due != 0xffffffffu)
-        atomicMin(out, due);
+    if ((threadIdx.x & 63) == 0)
+        block_due[threadIdx.x >> 6] = due;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        due = min(min(block_due[0], block_due[1]), min(block_due[2], block_due[3]));
+        if (due != 0xffffffffu)
+            atomicMin(out, due);
+    }
 }
 
 __global__ __launch_bounds__(256) void gather_u32_kernel(const uint32_t* __restrict__ src,
@@ -487,8 +522,8 @@ void launch_adaptive_next_due(Device& d, const AdaptiveParams& a, uint32_t t, ui
     launch_fill_u32(out, 0xffffffffu, 1, d.stream);
     if (d.n == 0)
         return;
-    adaptive_next_due_kernel<<<blocks_for(d.n), 256, 0, d.stream>>>(d.alevel, d.n, a.levels, t,
-                                                                     out);
+    adaptive_next_due_kernel<<<reduce_blocks(d.n), 256, 0, d.stream>>>(d.alevel, d.n, a.levels,
+                                                                        t, out);
 }
 
 void launch_adaptive_correct(Device& d, const AdaptiveParams& a, uint32_t tick,
@@ -496,7 +531,7 @@ void launch_adaptive_correct(Device& d, const AdaptiveParams& a, uint32_t tick,
 {
     if (d.n == 0)
         return;
-    adaptive_correct_kernel<<<blocks_for(d.n), 256, 0, d.stream>>>(
+    adaptive_correct_kernel<<<reduce_blocks(d.n), 256, 0, d.stream>>>(
         d.pos[0], d.pos[1], d.pos[2], d.vel[0], d.vel[1], d.vel[2], d.acc[0], d.acc[1], d.acc[2],
         d.aacc[0], d.aacc[1], d.aacc[2], d.alevel, d.atick, d.n, a, tick, count);
 }

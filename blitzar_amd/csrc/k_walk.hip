@@ -367,6 +367,10 @@ __global__ __launch_bounds__(64, FAR ? 1 : BH_WALK_MIN_WAVES) void walk_kernel(
     const float wfmax = fmaxf(fmaxf(wfx, wfy), wfz);
     const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     unsigned long long npp = 0, npc = 0;
+    // this wave's interaction totals, added to `stats` once when it exits
+    // (one set of atomics per wave instead of per group: the counters are
+    // single addresses every wave would otherwise contend for)
+    unsigned long long tot_pp = 0, tot_pc = 0, tot_mono = 0, tot_far = 0, tot_eval = 0;
     const int ngroups = info[kInfoGroups];
 
     for (;;) {
@@ -710,15 +714,13 @@ __global__ __launch_bounds__(64, FAR ? 1 : BH_WALK_MIN_WAVES) void walk_kernel(
         // per-target interaction counts for the roofline (only active lanes are real targets)
         npp = npp * static_cast<unsigned long long>(nact);
         npc = npc * static_cast<unsigned long long>(nact);
-        if (stats && lane == 0) {
-            atomicAdd(&stats[0], npp);
-            atomicAdd(&stats[1], npc);
-            if (CRIT == 3 && nmono)
-                atomicAdd(&stats[5], nmono * static_cast<unsigned long long>(nact));
-            if (FAR) {
-                atomicAdd(&stats[3], nfar);                                        // node-group
-                atomicAdd(&stats[4], static_cast<unsigned long long>(nact));      // evaluations
-            }
+        tot_pp += npp;
+        tot_pc += npc;
+        if (CRIT == 3)
+            tot_mono += nmono * static_cast<unsigned long long>(nact);
+        if (FAR) {
+            tot_far += nfar;                                      // node-group
+            tot_eval += static_cast<unsigned long long>(nact);   // evaluations
         }
         npp = 0;
         npc = 0;
@@ -740,6 +742,16 @@ __global__ __launch_bounds__(64, FAR ? 1 : BH_WALK_MIN_WAVES) void walk_kernel(
             accz[s] = az;
             if (P.amag)
                 P.amag[s] = sqrtf(ax * ax + ay * ay + az * az);
+        }
+    }
+    if (stats && lane == 0) {
+        atomicAdd(&stats[0], tot_pp);
+        atomicAdd(&stats[1], tot_pc);
+        if (CRIT == 3 && tot_mono)
+            atomicAdd(&stats[5], tot_mono);
+        if (FAR) {
+            atomicAdd(&stats[3], tot_far);
+            atomicAdd(&stats[4], tot_eval);
         }
     }
 }
