@@ -90,6 +90,15 @@ bool dalloc(bh_hip_handle* h, T** p, size_t count, const char* what)
     return true;
 }
 
+// dalloc, then zero the buffer in stream order (for workspaces whose state
+// must start at zero, such as the single-pass scans' status words)
+template <typename T>
+bool dalloc_zero(bh_hip_handle* h, T** p, size_t count, const char* what, hipStream_t s)
+{
+    return dalloc(h, p, count, what) &&
+           check(h, hipMemsetAsync(*p, 0, std::max<size_t>(count, 1) * sizeof(T), s), what);
+}
+
 void dfree(bh_hip_handle* h, void* p)
 {
     if (!p)
@@ -188,9 +197,11 @@ bool alloc_nodes(bh_hip_handle* h, size_t cap)
            dalloc(h, &d.wnode, cap, "walk nodes") && dalloc(h, &d.scan_tmp, scan_n, "scan tmp") &&
            dalloc(h, &d.bnd_tmp, 8 * scan_n, "child bounds") &&
            dalloc(h, &d.big_list, scan_n, "large-node list") &&
-           dalloc(h, &d.scan_blocks, bh::scan_block_sums_words(scan_n), "scan blocks") &&
+           dalloc_zero(h, &d.scan_blocks, bh::scan_block_sums_words(scan_n), "scan blocks",
+                       d.stream) &&
            dalloc(h, &d.nparent, cap, "node parent") && dalloc(h, &d.gscan, cap, "group scan") &&
-           dalloc(h, &d.gscan_blocks, bh::scan_block_sums_words(cap), "group scan blocks");
+           dalloc_zero(h, &d.gscan_blocks, bh::scan_block_sums_words(cap), "group scan blocks",
+                       d.stream);
 }
 
 bool alloc_all(bh_hip_handle* h, uint64_t n)
@@ -214,11 +225,11 @@ bool alloc_all(bh_hip_handle* h, uint64_t n)
         !dalloc(h, &d.groups, n, "groups"))
         return false;
     d.sort_hist_cap = bh::radix_hist_words(n);
-    if (!dalloc(h, &d.sort_hist, d.sort_hist_cap, "sort hist"))
+    if (!dalloc_zero(h, &d.sort_hist, d.sort_hist_cap, "sort hist", d.stream))
         return false;
     if (!dalloc(h, &d.level_info, 2 * bh::kLevels + 8, "level info") ||
         !dalloc(h, &d.geom, 1, "geometry") || !dalloc(h, &d.aabb_bits, 8, "aabb") ||
-        !dalloc(h, &d.err, 4, "error flags") || !dalloc(h, &d.work_counter, 4, "work counter") ||
+        !dalloc(h, &d.err, 4, "error flags") || !dalloc(h, &d.work_counter, bh::kTicketWords, "work counter") ||
         !dalloc(h, &h->stats, 6, "walk stats"))
         return false;
     d.diag_cap = 6 * 1024 + 8 * 256 + 16;  // AABB partials, energy partials
@@ -260,6 +271,8 @@ bh_hip_result_t finish(bh_hip_handle* h, const char* what)
             m += " traversal stack overflow;";
         if (flags & bh::kErrLeafTooLarge)
             m += " leaf larger than 2^23 particles;";
+        if (flags & bh::kErrScan)
+            m += " device scan did not complete (results of this call invalid);";
         h->last_error = m;
         if (flags & bh::kErrNodeOverflow)
             alloc_nodes(h, h->d.node_cap * 2);
@@ -1074,15 +1087,23 @@ bh_hip_result_t bh_hip_sort_pairs(const uint64_t* keys, const uint32_t* vals, ui
     d.sort_hist_cap = bh::radix_hist_words(n);
     ok = ok && dalloc(&tmp, &d.keys, n, "keys") && dalloc(&tmp, &d.keys_b, n, "keys_b") &&
          dalloc(&tmp, &d.vals, n, "vals") && dalloc(&tmp, &d.vals_b, n, "vals_b") &&
-         dalloc(&tmp, &d.sort_hist, d.sort_hist_cap, "hist");
+         dalloc_zero(&tmp, &d.sort_hist, d.sort_hist_cap, "hist", d.stream) &&
+         dalloc_zero(&tmp, &d.err, 4, "error flags", d.stream) &&
+         check(&tmp, hipStreamSynchronize(d.stream), "hist");
     ok = ok && check(&tmp, hipMemcpy(d.keys, keys, n * 8, hipMemcpyHostToDevice), "keys in") &&
          check(&tmp, hipMemcpy(d.vals, vals, n * 4, hipMemcpyHostToDevice), "vals in");
     if (ok) {
         bh::launch_radix_sort(d, n, key_bits);
+        int flags = 0;
         ok = check(&tmp, hipGetLastError(), "sort") &&
              check(&tmp, hipStreamSynchronize(d.stream), "sort") &&
+             check(&tmp, hipMemcpy(&flags, d.err, sizeof(int), hipMemcpyDeviceToHost), "sort") &&
              check(&tmp, hipMemcpy(keys_out, d.keys, n * 8, hipMemcpyDeviceToHost), "keys out") &&
              check(&tmp, hipMemcpy(vals_out, d.vals, n * 4, hipMemcpyDeviceToHost), "vals out");
+        if (ok && flags != 0) {
+            tmp.last_error = "[octree_hip] sort: scan did not complete";
+            ok = false;
+        }
     }
     for (void* p : tmp.allocations)
         (void)hipFree(p);

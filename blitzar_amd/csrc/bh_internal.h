@@ -17,6 +17,11 @@ constexpr int kInfoTotal = 2 * kLevels;      // level_info slot: total nodes
 constexpr int kInfoMaxLevel = 2 * kLevels + 1;  // deepest populated level
 constexpr int kInfoGroups = 2 * kLevels + 2;    // number of walk groups
 constexpr int kInfoBig = 2 * kLevels + 3;       // large nodes of the current level (tree build)
+// Walk tickets: one counter per XCD (workgroup i runs on XCD i mod 8), each
+// on its own 256-byte line.
+constexpr int kTicketQueues = 8;
+constexpr int kTicketStride = 64;  // uint32 words between counters
+constexpr int kTicketWords = kTicketQueues * kTicketStride;
 
 // Learned far-field MLP: 16 inputs (15 features + pad) -> 64 -> 64 -> 3.
 constexpr int kLffInputs = 16;
@@ -98,6 +103,7 @@ enum DeviceError : int {
     kErrStackOverflow = 2,
     kErrLeafTooLarge = 4,
     kErrKeyMode = 8,
+    kErrScan = 16,  // a scan block gave up waiting for its predecessors
 };
 
 // Parameters the walk needs (by value into the kernel).
@@ -255,11 +261,12 @@ void launch_keys(Device& d, int key_mode);
 // bits [0, key_bits). Result lands back in d.keys / d.vals.
 void launch_radix_sort(Device& d, size_t n, int key_bits);
 size_t radix_hist_words(size_t n);
-// Exclusive scan of `data` in place. The element count is *n_dev when n_dev
-// is non-null (then max_n bounds it), else n_host. The total is written to
-// total_out / total_out_i when non-null.
+// Exclusive scan of `data` in place (one launch). The element count is *n_dev
+// when n_dev is non-null (then max_n bounds it), else n_host. The total is
+// written to total_out / total_out_i when non-null. `block_sums` is the scan's
+// workspace of scan_block_sums_words(max_n) words, zeroed once when allocated.
 void launch_scan_exclusive(uint32_t* data, size_t max_n, const int* n_dev, size_t n_host,
-                           uint32_t* block_sums, uint32_t* total_out, int* total_out_i,
+                           uint32_t* block_sums, uint32_t* total_out, int* total_out_i, int* err,
                            hipStream_t s);
 size_t scan_block_sums_words(size_t max_n);
 void launch_tie_fix(Device& d);

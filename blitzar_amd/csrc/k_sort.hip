@@ -8,11 +8,12 @@
 // tie_fix_kernel then reorders every run of equal keys by particle id, which
 // restores the reference contract exactly.
 //
-// One pass (8-bit digit) = histogram kernel (reads 8 B/key), one scan of the
-// digit-major tile counts, and a scatter kernel that ranks keys inside a tile
+// One pass (8-bit digit) = histogram kernel (reads 8 B/key), one single-pass
+// scan of the digit-major tile counts, and a scatter kernel that ranks keys inside a tile
 // of 2048 with wave ballots, reorders the tile in LDS and writes digit runs
 // contiguously (reads 12 B/key, writes 12 B/key).
 #include "bh_internal.h"
+#include "scan_lookback.h"
 
 namespace bh {
 namespace {
@@ -22,48 +23,10 @@ constexpr int kItems = 8;
 constexpr int kTile = kBlock * kItems;  // 2048
 constexpr int kWaves = kBlock / 64;
 
-constexpr int kScanBlock = 256;
-constexpr int kScanItems = 8;
-constexpr int kScanChunk = kScanBlock * kScanItems;  // 2048
-
 __device__ inline unsigned long long lanemask_lt()
 {
     const unsigned lane = threadIdx.x & 63u;
     return lane == 0 ? 0ull : (~0ull >> (64u - lane));
-}
-
-// Inclusive scan over one wave.
-__device__ inline uint32_t wave_inclusive(uint32_t v)
-{
-    const int lane = threadIdx.x & 63;
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t u = __shfl_up(v, o, 64);
-        if (lane >= o)
-            v += u;
-    }
-    return v;
-}
-
-// Exclusive scan over a 256-thread block; returns the exclusive prefix and
-// the block total in *total. `tmp` must hold kWaves words.
-__device__ inline uint32_t block_exclusive256(uint32_t v, uint32_t* tmp, uint32_t* total)
-{
-    const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
-    const uint32_t inc = wave_inclusive(v);
-    if (lane == 63)
-        tmp[w] = inc;
-    __syncthreads();
-    uint32_t off = 0, all = 0;
-    for (int k = 0; k < kWaves; ++k) {
-        const uint32_t s = tmp[k];
-        if (k < w)
-            off += s;
-        all += s;
-    }
-    __syncthreads();
-    *total = all;
-    return off + inc - v;
 }
 
 __global__ __launch_bounds__(kBlock) void hist_kernel(const unsigned long long* __restrict__ keys,
@@ -182,97 +145,18 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(
 }
 
 // ---- device-wide exclusive scan (element count from host or device) -------
-__device__ inline size_t scan_count(const int* n_dev, size_t n_host)
+__global__ __launch_bounds__(kScanBlock) void scan_lookback_kernel(
+    uint32_t* __restrict__ data, const int* n_dev, size_t n_host, uint32_t* __restrict__ ws,
+    uint32_t* total_out, int* total_out_i, int* err)
 {
-    return n_dev ? static_cast<size_t>(*n_dev) : n_host;
-}
-
-__global__ __launch_bounds__(kScanBlock) void scan_reduce_kernel(const uint32_t* __restrict__ in,
-                                                                 const int* n_dev, size_t n_host,
-                                                                 uint32_t* __restrict__ sums)
-{
-    __shared__ uint32_t red[kScanBlock / 64];
-    const size_t n = scan_count(n_dev, n_host);
-    const size_t base = static_cast<size_t>(blockIdx.x) * kScanChunk;
-    if (base >= n)
-        return;
-    uint32_t s = 0;
-    for (int k = 0; k < kScanItems; ++k) {
-        const size_t i = base + static_cast<size_t>(threadIdx.x) * kScanItems + k;
-        if (i < n)
-            s += in[i];
-    }
-    uint32_t total;
-    block_exclusive256(s, red, &total);
-    if (threadIdx.x == 0)
-        sums[blockIdx.x] = total;
-}
-
-__global__ __launch_bounds__(1024) void scan_sums_kernel(uint32_t* sums, const int* n_dev,
-                                                         size_t n_host, uint32_t* total_out,
-                                                         int* total_out_i)
-{
-    __shared__ uint32_t wsum[16];
-    __shared__ uint32_t carry;
-    const size_t n = scan_count(n_dev, n_host);
-    const size_t nb = (n + kScanChunk - 1) / kScanChunk;
-    const int t = threadIdx.x;
-    const int lane = t & 63, w = t >> 6;
-    if (t == 0)
-        carry = 0;
-    __syncthreads();
-    for (size_t base = 0; base < nb; base += 1024) {
-        const size_t i = base + t;
-        const uint32_t v = i < nb ? sums[i] : 0u;
-        const uint32_t inc = wave_inclusive(v);
-        if (lane == 63)
-            wsum[w] = inc;
-        __syncthreads();
-        uint32_t off = carry;
-        uint32_t all = 0;
-        for (int k = 0; k < 16; ++k) {
-            if (k < w)
-                off += wsum[k];
-            all += wsum[k];
-        }
-        if (i < nb)
-            sums[i] = off + inc - v;
-        __syncthreads();
-        if (t == 0)
-            carry += all;
-        __syncthreads();
-    }
-    if (t == 0) {
+    size_t n = n_dev ? static_cast<size_t>(*n_dev > 0 ? *n_dev : 0) : n_host;
+    n = n < static_cast<size_t>(gridDim.x) * kScanChunk ? n : static_cast<size_t>(gridDim.x) * kScanChunk;
+    const ScanBlockResult r = scan_lookback_block(data, n, ws, err, kErrScan);
+    if (r.last && threadIdx.x == 0) {
         if (total_out)
-            *total_out = carry;
+            *total_out = r.total;
         if (total_out_i)
-            *total_out_i = static_cast<int>(carry);
-    }
-}
-
-__global__ __launch_bounds__(kScanBlock) void scan_down_kernel(uint32_t* __restrict__ data,
-                                                               const int* n_dev, size_t n_host,
-                                                               const uint32_t* __restrict__ sums)
-{
-    __shared__ uint32_t red[kScanBlock / 64];
-    const size_t n = scan_count(n_dev, n_host);
-    const size_t base = static_cast<size_t>(blockIdx.x) * kScanChunk;
-    if (base >= n)
-        return;
-    uint32_t v[kScanItems];
-    uint32_t s = 0;
-    for (int k = 0; k < kScanItems; ++k) {
-        const size_t i = base + static_cast<size_t>(threadIdx.x) * kScanItems + k;
-        v[k] = i < n ? data[i] : 0u;
-        s += v[k];
-    }
-    uint32_t total;
-    uint32_t off = block_exclusive256(s, red, &total) + sums[blockIdx.x];
-    for (int k = 0; k < kScanItems; ++k) {
-        const size_t i = base + static_cast<size_t>(threadIdx.x) * kScanItems + k;
-        if (i < n)
-            data[i] = off;
-        off += v[k];
+            *total_out_i = static_cast<int>(r.total);
     }
 }
 
@@ -365,17 +249,16 @@ __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, size_t n)
 
 }  // namespace
 
-size_t scan_block_sums_words(size_t max_n) { return (max_n + kScanChunk - 1) / kScanChunk + 1; }
+size_t scan_block_sums_words(size_t max_n) { return scan_workspace_words(max_n); }
 
 void launch_scan_exclusive(uint32_t* data, size_t max_n, const int* n_dev, size_t n_host,
                            uint32_t* block_sums, uint32_t* total_out, int* total_out_i,
-                           hipStream_t s)
+                           int* err, hipStream_t s)
 {
     const size_t cap = n_dev ? max_n : n_host;
     const unsigned nb = static_cast<unsigned>(std::max<size_t>(1, (cap + kScanChunk - 1) / kScanChunk));
-    scan_reduce_kernel<<<nb, kScanBlock, 0, s>>>(data, n_dev, n_host, block_sums);
-    scan_sums_kernel<<<1, 1024, 0, s>>>(block_sums, n_dev, n_host, total_out, total_out_i);
-    scan_down_kernel<<<nb, kScanBlock, 0, s>>>(data, n_dev, n_host, block_sums);
+    scan_lookback_kernel<<<nb, kScanBlock, 0, s>>>(data, n_dev, n_host, block_sums, total_out,
+                                                   total_out_i, err);
 }
 
 size_t radix_hist_words(size_t n)
@@ -403,7 +286,7 @@ void launch_radix_sort(Device& d, size_t n, int key_bits)
         const int bits = key_bits - shift < 8 ? key_bits - shift : 8;
         const uint32_t dmask = (1u << bits) - 1u;
         hist_kernel<<<ntiles, kBlock, 0, d.stream>>>(kin, n, shift, dmask, hist, ntiles);
-        launch_scan_exclusive(hist, hn, nullptr, hn, sums, nullptr, nullptr, d.stream);
+        launch_scan_exclusive(hist, hn, nullptr, hn, sums, nullptr, nullptr, d.err, d.stream);
         scatter_kernel<<<ntiles, kBlock, 0, d.stream>>>(kin, vin, kout, vout, n, shift, dmask,
                                                         hist, ntiles);
         std::swap(kin, kout);

@@ -22,6 +22,7 @@
 // children with the parallel-axis theorem, using exact integer differences of
 // lattice centres.
 #include "bh_internal.h"
+#include "scan_lookback.h"
 
 namespace bh {
 namespace {
@@ -257,28 +258,43 @@ __global__ __launch_bounds__(256) void tree_count_big_kernel(
     }
 }
 
-// After the scan of level `level`: place level+1, clamp to capacity.
-__global__ void tree_level_fix_kernel(int* info, int level, int node_cap, int* err)
+// After the scan of level `level` (whose total is the node count of level+1):
+// place level+1, clamp to capacity.
+__device__ inline void level_fix(int* info, int level, int next_count, int node_cap, int* err)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0)
-        return;
     int* base = level_base(info);
     int* count = level_count(info);
     const int nb = base[level] + count[level];
     base[level + 1] = nb;
-    int c = count[level + 1];
+    int c = next_count;
     if (nb + c > node_cap) {
         // Out of node slots: level+1 is dropped and every node of `level` that
         // wanted to split stays a leaf (see tree_emit_kernel). Forces remain
         // correct; the host grows the capacity for the next build.
         atomicOr(err, kErrNodeOverflow);
         c = 0;
-        count[level + 1] = 0;
     }
+    count[level + 1] = c;
     if (c > 0)
         info[2 * kLevels + 1] = level + 1;  // deepest populated level
     info[2 * kLevels] = nb + c;             // total nodes
     info[kInfoBig] = 0;                     // large-node list of the next level
+}
+
+// Exclusive scan of the child counts of `level` (scan_tmp, one word per node
+// of the level) in one pass, then level_fix in the block that ends the scan.
+__global__ __launch_bounds__(kScanBlock) void tree_scan_fix_kernel(uint32_t* __restrict__ scan_tmp,
+                                                                   uint32_t* __restrict__ ws,
+                                                                   int* info, int level,
+                                                                   int node_cap, int* err)
+{
+    const int cnt = level_count(info)[level];
+    size_t n = static_cast<size_t>(cnt > 0 ? cnt : 0);
+    const size_t cap = static_cast<size_t>(gridDim.x) * kScanChunk;
+    n = n < cap ? n : cap;
+    const ScanBlockResult r = scan_lookback_block(scan_tmp, n, ws, err, kErrScan);
+    if (r.last && threadIdx.x == 0)
+        level_fix(info, level, static_cast<int>(r.total), node_cap, err);
 }
 
 // Children of the nodes of `level` from the bounds tree_count_kernel left in
@@ -594,9 +610,10 @@ void launch_tree_build(Device& d, int leaf_depth, int bucket)
             tree_count_big_kernel<<<64, 256, 0, d.stream>>>(keys, d.nstart, d.ncount, d.nchild,
                                                             d.nfirst, d.scan_tmp, d.bnd_tmp,
                                                             d.big_list, d.level_info, L);
-        launch_scan_exclusive(d.scan_tmp, bound, d.level_info + kLevels + L, 0, d.scan_blocks,
-                              nullptr, d.level_info + kLevels + L + 1, d.stream);
-        tree_level_fix_kernel<<<1, 64, 0, d.stream>>>(d.level_info, L, cap, d.err);
+        const unsigned nscan =
+            static_cast<unsigned>(std::max<size_t>(1, (bound + kScanChunk - 1) / kScanChunk));
+        tree_scan_fix_kernel<<<nscan, kScanBlock, 0, d.stream>>>(d.scan_tmp, d.scan_blocks,
+                                                                 d.level_info, L, cap, d.err);
         tree_emit_kernel<<<level_grid(bound), 256, 0, d.stream>>>(
             d.nprefix, d.nlevel, d.nstart, d.ncount, d.nchild, d.nfirst, d.nparent, d.scan_tmp,
             d.bnd_tmp, d.level_info, L);
@@ -609,7 +626,7 @@ void launch_groups(Device& d)
     group_count_kernel<<<g, 256, 0, d.stream>>>(d.nprefix, d.nstart, d.ncount, d.nchild, d.nfirst,
                                                 d.nparent, d.level_info, d.gscan);
     launch_scan_exclusive(d.gscan, d.node_cap, d.level_info + kInfoTotal, 0, d.gscan_blocks,
-                          nullptr, d.level_info + kInfoGroups, d.stream);
+                          nullptr, d.level_info + kInfoGroups, d.err, d.stream);
     group_emit_kernel<<<g, 256, 0, d.stream>>>(d.nprefix, d.nstart, d.ncount, d.nchild, d.nfirst,
                                                d.nparent, d.gscan, d.level_info, d.groups);
 }

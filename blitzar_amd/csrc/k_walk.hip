@@ -56,6 +56,31 @@ namespace {
 constexpr int kCellCap = BH_WALK_CELL_CAP;
 constexpr int kPartCap = BH_WALK_PART_CAP;
 
+// Group tickets. On large walks (at least kQueueMinGroupsPerWave groups per
+// resident wave) the walk order is cut into chunks of kTicketChunk
+// consecutive groups, dealt round-robin to kQueues counters; a wave draws
+// from the queue of its XCD, so the groups one XCD walks at a time are
+// neighbours in space and share tree nodes in that XCD's L2. Smaller walks
+// use one counter: with few groups per wave, the per-XCD split costs more in
+// balance than it gains in locality.
+#ifndef BH_WALK_TICKET_QUEUES
+#define BH_WALK_TICKET_QUEUES 8
+#endif
+#ifndef BH_WALK_TICKET_CHUNK
+#define BH_WALK_TICKET_CHUNK 16
+#endif
+constexpr int kQueues = BH_WALK_TICKET_QUEUES;
+constexpr unsigned kTicketChunk = BH_WALK_TICKET_CHUNK;
+constexpr int kQueueMinGroupsPerWave = 16;
+static_assert(kQueues >= 1 && kQueues <= kTicketQueues, "ticket queues");
+
+// position in the walk order of the k-th ticket of queue q (of nq)
+__device__ inline int ticket_position(unsigned k, int q, int nq)
+{
+    return static_cast<int>(((k / kTicketChunk) * static_cast<unsigned>(nq) +
+                             static_cast<unsigned>(q)) * kTicketChunk + k % kTicketChunk);
+}
+
 __device__ inline float wmin(float v)
 {
     for (int o = 32; o > 0; o >>= 1)
@@ -356,7 +381,6 @@ __global__ __launch_bounds__(64, FAR ? 1 : BH_WALK_MIN_WAVES) void walk_kernel(
     __shared__ int lfFirst[64];
     __shared__ uint32_t lfPre[64];
     __shared__ float lfSx[64], lfSy[64], lfSz[64];
-    __shared__ int sGroup;
     __shared__ uint32_t fl[FAR ? kFarCap : 1];
     __shared__ float fsum[FAR ? kFarTerms : 1];
 
@@ -373,12 +397,31 @@ __global__ __launch_bounds__(64, FAR ? 1 : BH_WALK_MIN_WAVES) void walk_kernel(
     unsigned long long tot_pp = 0, tot_pc = 0, tot_mono = 0, tot_far = 0, tot_eval = 0;
     const int ngroups = info[kInfoGroups];
 
+    // A wave draws from its XCD's queue and, once that is drained, from the
+    // others in turn; every queue position below ngroups is handed out exactly
+    // once. (A ticket is drawn only when the wave is free: a wave that held
+    // its next group while walking the current one would keep that group from
+    // an idle wave at the end of the walk.)
+    const int nq = static_cast<long long>(ngroups) >=
+                           static_cast<long long>(kQueueMinGroupsPerWave) * gridDim.x
+                       ? kQueues
+                       : 1;
+    int queue = static_cast<int>(blockIdx.x % static_cast<unsigned>(nq));
+    int drained = 0;
     for (;;) {
-        if (lane == 0)
-            sGroup = static_cast<int>(atomicAdd(work, 1u));
-        __syncthreads();
-        const int ticket = sGroup;
-        __syncthreads();
+        int t = ngroups;
+        if (lane == 0) {
+            for (;;) {
+                t = ticket_position(atomicAdd(&work[queue * kTicketStride], 1u), queue, nq);
+                if (t < ngroups || drained == nq - 1)
+                    break;
+                ++drained;
+                queue = (queue + 1) % nq;
+            }
+            t = t < ngroups ? t : ngroups;
+        }
+        // the whole wave is active here, so the first lane is lane 0
+        const int ticket = __builtin_amdgcn_readfirstlane(t);
         if (ticket >= ngroups)
             break;
         // Groups are stored in the breadth-first order of their nodes, shallow
@@ -776,7 +819,7 @@ void launch_walk_record(Device& d, const WalkParams& p, unsigned long long* stat
 {
     if (d.n == 0)
         return;
-    launch_fill_u32(d.work_counter, 0u, 1, d.stream);
+    launch_fill_u32(d.work_counter, 0u, kTicketWords, d.stream);
     const unsigned grid = static_cast<unsigned>(d.stack_waves);
     if (p.criterion == 1)
         launch_walk_t<2, 1, false, true>(d, p, grid, stats, &L, band, min_count);
@@ -790,7 +833,7 @@ void launch_walk(Device& d, const WalkParams& p, unsigned long long* stats)
 {
     if (d.n == 0)
         return;
-    launch_fill_u32(d.work_counter, 0u, 1, d.stream);
+    launch_fill_u32(d.work_counter, 0u, kTicketWords, d.stream);
     const unsigned grid = static_cast<unsigned>(d.stack_waves);
     const bool tpm = p.rs > 0.0f;
     if (p.far2 > 0.0f && !tpm && p.order >= 2) {
