@@ -10,7 +10,7 @@
 //
 // One pass (8-bit digit) = histogram kernel (reads 8 B/key), one single-pass
 // scan of the digit-major tile counts, and a scatter kernel that ranks keys inside a tile
-// of 2048 with wave ballots, reorders the tile in LDS and writes digit runs
+// of 4096 with wave ballots, reorders the tile in LDS and writes digit runs
 // contiguously (reads 12 B/key, writes 12 B/key).
 #include "bh_internal.h"
 #include "scan_lookback.h"
@@ -19,8 +19,14 @@ namespace bh {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kItems = 8;
-constexpr int kTile = kBlock * kItems;  // 2048
+#ifndef BH_SORT_ITEMS
+#define BH_SORT_ITEMS 16
+#endif
+// Keys per thread. A tile of 4096 keys (16 per thread) writes digit runs of 16
+// keys on average, twice those of a 2048-key tile: fewer partial cache lines
+// (70 us less per 8M-key sort than 8 per thread, at 2 blocks per CU).
+constexpr int kItems = BH_SORT_ITEMS;
+constexpr int kTile = kBlock * kItems;  // keys per tile
 constexpr int kWaves = kBlock / 64;
 
 __device__ inline unsigned long long lanemask_lt()
