@@ -57,7 +57,8 @@ def test_sorted_order_ties_by_particle_index(oracle):
 
 
 @pytest.mark.parametrize("n,leaf_cap,bucket", [(1, 256, 16), (5, 256, 16), (3000, 16, 8),
-                                               (40000, 256, 16), (40000, 16, 1), (40000, 64, 64)])
+                                               (40000, 256, 16), (40000, 16, 1), (40000, 64, 64),
+                                               (1500000, 16, 1)])
 def test_topology_bit_exact(oracle, n, leaf_cap, bucket):
     say(f"case topology n={n} leaf_cap={leaf_cap} bucket={bucket}")
     pos, m = _clustered(n, 200 + n)
@@ -145,13 +146,18 @@ def test_sort_pairs_edge_cases():
     cases = [
         ("single", np.array([5], np.uint64)),
         ("pair", np.array([9, 3], np.uint64)),
-        ("tile-1", rng.integers(0, 2 ** 63, 2047, dtype=np.uint64)),
-        ("tile", rng.integers(0, 2 ** 63, 2048, dtype=np.uint64)),
-        ("tile+1", rng.integers(0, 2 ** 63, 2049, dtype=np.uint64)),
+        ("2047", rng.integers(0, 2 ** 63, 2047, dtype=np.uint64)),
+        ("2049", rng.integers(0, 2 ** 63, 2049, dtype=np.uint64)),
+        ("tile-1", rng.integers(0, 2 ** 63, 4095, dtype=np.uint64)),
+        ("tile", rng.integers(0, 2 ** 63, 4096, dtype=np.uint64)),
+        ("tile+1", rng.integers(0, 2 ** 63, 4097, dtype=np.uint64)),
         ("dups", rng.integers(0, 7, 100000).astype(np.uint64)),
         ("equal", np.full(30000, 12345, np.uint64)),
         ("max", np.full(5000, 2 ** 64 - 1, np.uint64)),
         ("big", rng.integers(0, 2 ** 64 - 1, 300000, dtype=np.uint64)),
+        # 733 tiles: the digit-count scan spans 92 blocks, so its look-back
+        # crosses more than one 64-block window
+        ("huge", rng.integers(0, 2 ** 64 - 1, 3000000, dtype=np.uint64)),
     ]
     for name, k in cases:
         say(f"case sort {name} n={k.size}")
