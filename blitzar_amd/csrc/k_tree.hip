@@ -404,34 +404,41 @@ __global__ __launch_bounds__(256) void tree_moments_kernel(
             bmax = sqrtf(bmax);
         } else {
             float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-            for (uint32_t j = first; j < first + nc; ++j) {
-                const WalkNode ch = wn[j];
+            // The (at most 8) children unrolled, each load in bounds (index
+            // clamped to the last child, whose repeats weigh 0), so that the
+            // loads issue together instead of one round trip per child.
+#pragma unroll
+            for (uint32_t jj = 0; jj < 8; ++jj) {
+                const WalkNode ch = wn[first + (jj < nc ? jj : nc - 1u)];
+                const float cm = jj < nc ? ch.mass : 0.0f;
                 const float rx = static_cast<float>(ch.cx - c[0]) * ux + ch.comx;
                 const float ry = static_cast<float>(ch.cy - c[1]) * uy + ch.comy;
                 const float rz = static_cast<float>(ch.cz - c[2]) * uz + ch.comz;
-                M += ch.mass;
-                sx += ch.mass * rx;
-                sy += ch.mass * ry;
-                sz += ch.mass * rz;
+                M += cm;
+                sx += cm * rx;
+                sy += cm * ry;
+                sz += cm * rz;
             }
             if (M > 0.0f) {
                 cx = sx / M;
                 cy = sy / M;
                 cz = sz / M;
             }
-            for (uint32_t j = first; j < first + nc; ++j) {
-                const WalkNode ch = wn[j];
+#pragma unroll
+            for (uint32_t jj = 0; jj < 8; ++jj) {
+                const bool live = jj < nc;
+                const WalkNode ch = wn[first + (live ? jj : nc - 1u)];
                 const float dx = static_cast<float>(ch.cx - c[0]) * ux + ch.comx - cx;
                 const float dy = static_cast<float>(ch.cy - c[1]) * uy + ch.comy - cy;
                 const float dz = static_cast<float>(ch.cz - c[2]) * uz + ch.comz - cz;
                 const float r2 = dx * dx + dy * dy + dz * dz;
-                qxx += ch.qxx + ch.mass * (3.0f * dx * dx - r2);
-                qxy += ch.qxy + ch.mass * (3.0f * dx * dy);
-                qxz += ch.qxz + ch.mass * (3.0f * dx * dz);
-                qyy += ch.qyy + ch.mass * (3.0f * dy * dy - r2);
-                qyz += ch.qyz + ch.mass * (3.0f * dy * dz);
-                trs += ch.trs + ch.mass * r2;
-                if (ch.mass > 0.0f)
+                qxx += live ? ch.qxx + ch.mass * (3.0f * dx * dx - r2) : 0.0f;
+                qxy += live ? ch.qxy + ch.mass * (3.0f * dx * dy) : 0.0f;
+                qxz += live ? ch.qxz + ch.mass * (3.0f * dx * dz) : 0.0f;
+                qyy += live ? ch.qyy + ch.mass * (3.0f * dy * dy - r2) : 0.0f;
+                qyz += live ? ch.qyz + ch.mass * (3.0f * dy * dz) : 0.0f;
+                trs += live ? ch.trs + ch.mass * r2 : 0.0f;
+                if (live && ch.mass > 0.0f)
                     bmax = fmaxf(bmax, sqrtf(r2) + ch.bmax);
             }
         }
