@@ -200,3 +200,31 @@ def test_topology_with_a_dense_clump(oracle):
     assert len(t["level"]) == len(ref["level"])
     for key in ("level", "prefix", "start", "count", "first_child", "nchild"):
         assert np.array_equal(t[key].astype(np.int64), ref[key].astype(np.int64)), key
+
+
+def test_node_capacity_overflow_recovers(oracle):
+    """Near-duplicate pairs split in single-child chains down to level 20:
+    16,153 nodes for 2,000 particles, twice the initial capacity (2n + 4096).
+    The first build drops the level that would overflow (flagged, with the
+    level above standing in as leaves; every write stays inside the node
+    arrays), the capacity doubles, and the next build matches the oracle
+    exactly."""
+    import blitzar_amd as B
+    n = 2000
+    rng = np.random.default_rng(9)
+    base = rng.random((n // 2, 3))
+    pos = np.repeat(base, 2, axis=0)
+    pos[1::2] += 1.0e-6
+    m = np.full(n, 1.0 / n, np.float32)
+    say("case node capacity overflow n=2000")
+    with _solver(n, leaf_capacity=16, bucket_capacity=1) as s:
+        s.upload(pos, None, m)
+        with pytest.raises(B.BhHipError, match="node capacity"):
+            s.build_tree()
+        s.build_tree()
+        t = s.dump_tree()
+    sk, _ = oracle.stable_sort(oracle.keys(pos, 0))
+    ref = oracle.hip_tree(sk, oracle.leaf_depth(n, 16), 1)
+    assert len(t["level"]) == len(ref["level"]) == 16153
+    for key in ("level", "prefix", "start", "count", "first_child", "nchild"):
+        assert np.array_equal(t[key].astype(np.int64), ref[key].astype(np.int64)), key
