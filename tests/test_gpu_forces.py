@@ -374,3 +374,32 @@ def test_relative_point_mass_cells():
     assert out[1.0][2] < out[0.1][2] < out[0.0][2]
     assert out[0.1][0] < 1.3 * out[0.0][0] + 1e-5
     assert out[1.0][0] < 1e-2
+
+
+@pytest.mark.parametrize("criterion", ["com", "relative"])
+def test_ticket_queues_do_not_change_results(criterion):
+    """Walks of at least 16 groups per resident wave hand groups out through
+    eight per-XCD ticket queues, smaller walks through one counter. Every
+    group is walked exactly once either way, and a group's forces do not
+    depend on the wave that walks it: a residency of 2 waves per CU (queues)
+    and the default (one counter) give bit-identical accelerations, also
+    after steps that permute the store."""
+    import blitzar_amd as B
+    n = 600000
+    say(f"case ticket queues criterion={criterion} n={n}")
+    pos, vel, m = B.plummer(n, 21)
+    out = []
+    for wpc in (2, 24):
+        with B.OctreeHip(n, theta=0.6, softening=0.005, criterion=criterion,
+                         relative_alpha=0.004, waves_per_cu=wpc) as s:
+            s.set_timing(True)
+            s.upload(pos, vel, m)
+            a0 = s.compute_accelerations()
+            s.step(1e-3, 2)
+            a1 = s.compute_accelerations()
+            groups = s.timings()["groups"]
+            out.append((a0, a1, s.download()[0]))
+        say(f"  waves/CU {wpc}: {groups} groups")
+    assert groups >= 16 * 2 * 256 > 0  # the residency-2 walk used the queues
+    for x, y in zip(out[0], out[1]):
+        assert np.array_equal(x, y)
