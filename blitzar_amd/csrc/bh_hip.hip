@@ -854,6 +854,17 @@ bh_hip_result_t bh_hip_step(bh_hip_handle_t* h, double dt, int32_t nsteps)
     bh::Device& d = h->d;
     const double vmax = h->p.max_speed;
     float int_ms = 0.0f;
+    auto time_integration = [&]() {
+        if (h->timing && hipEventSynchronize(h->i1) == hipSuccess &&
+            hipEventElapsedTime(&int_ms, h->i0, h->i1) == hipSuccess)
+            h->integrate_ms += int_ms;
+    };
+    // Leapfrog: the closing half kick of a step and the opening half kick of
+    // the next use the same acceleration, so without a velocity clamp they
+    // are applied as one kick of dt (v + a dt instead of (v + a dt/2) + a dt/2,
+    // which differ by rounding only). The last step of the call keeps its
+    // closing half kick: velocities leave the call at whole steps.
+    bool half_kick_due = false;
     for (int k = 0; k < nsteps; ++k) {
         if (h->p.integrator == BH_HIP_INTEGRATOR_EULER) {
             queue_force(h);
@@ -861,28 +872,22 @@ bh_hip_result_t bh_hip_step(bh_hip_handle_t* h, double dt, int32_t nsteps)
             bh::launch_kick_drift(d, dt, dt, vmax, h->p.periodic_box);
             rec(h->timing ? h->i1 : nullptr, d.stream);
             h->primed = false;
+            time_integration();
         } else {
             if (!h->primed)
                 queue_force(h);
             rec(h->timing ? h->i0 : nullptr, d.stream);
-            bh::launch_kick_drift(d, 0.5 * dt, dt, vmax, h->p.periodic_box);
+            bh::launch_kick_drift(d, half_kick_due ? dt : 0.5 * dt, dt, vmax, h->p.periodic_box);
             rec(h->timing ? h->i1 : nullptr, d.stream);
-            if (h->timing) {
-                // integration time of this step: the kick-drift kernel (the
-                // closing half kick is timed below in the same way)
-                if (hipEventSynchronize(h->i1) == hipSuccess &&
-                    hipEventElapsedTime(&int_ms, h->i0, h->i1) == hipSuccess)
-                    h->integrate_ms += int_ms;
-            }
+            time_integration();
             queue_force(h);
-            rec(h->timing ? h->i0 : nullptr, d.stream);
-            bh::launch_kick(d, 0.5 * dt, vmax);
-            rec(h->timing ? h->i1 : nullptr, d.stream);
-        }
-        if (h->timing) {
-            if (hipEventSynchronize(h->i1) == hipSuccess &&
-                hipEventElapsedTime(&int_ms, h->i0, h->i1) == hipSuccess)
-                h->integrate_ms += int_ms;
+            half_kick_due = vmax <= 0.0 && k + 1 < nsteps;
+            if (!half_kick_due) {
+                rec(h->timing ? h->i0 : nullptr, d.stream);
+                bh::launch_kick(d, 0.5 * dt, vmax);
+                rec(h->timing ? h->i1 : nullptr, d.stream);
+                time_integration();
+            }
         }
     }
     const bh_hip_result_t r = finish(h, "step");
