@@ -17,6 +17,9 @@ constexpr int kInfoTotal = 2 * kLevels;      // level_info slot: total nodes
 constexpr int kInfoMaxLevel = 2 * kLevels + 1;  // deepest populated level
 constexpr int kInfoGroups = 2 * kLevels + 2;    // number of walk groups
 constexpr int kInfoBig = 2 * kLevels + 3;       // large nodes of the current level (tree build)
+// AABB: per-block partial extrema (6 doubles per block) in Device::diag.
+constexpr int kAabbBlocks = 1024;
+
 // Walk tickets: one counter per XCD (workgroup i runs on XCD i mod 8), each
 // on its own 256-byte line.
 constexpr int kTicketQueues = 8;
@@ -164,6 +167,10 @@ struct Device {
     float* amag = nullptr;    // |a| of the last force evaluation (relative criterion)
     float* amag_b = nullptr;
     bool amag_live = false;   // amag is in use: the store permutation carries it
+    // > 0: the last drift left this many per-block AABB partials of the
+    // current positions in diag (the next build skips its AABB pass); every
+    // other writer of positions or diag clears it
+    int aabb_parts = 0;
 
     uint64_t* keys = nullptr;
     uint64_t* keys_b = nullptr;

@@ -16,6 +16,8 @@
 // Energies / momenta: engine/physics/fmm/metrics/FmmMetrics.cpp:47-83.
 #include "bh_internal.h"
 
+#include <cfloat>
+
 namespace bh {
 namespace {
 
@@ -59,6 +61,75 @@ __global__ __launch_bounds__(256) void kick_drift_kernel(
         px[i] = wrap_box(px[i] + x * dtd, box);
         py[i] = wrap_box(py[i] + y * dtd, box);
         pz[i] = wrap_box(pz[i] + z * dtd, box);
+    }
+}
+
+__device__ inline double wave_min_d(double v)
+{
+    for (int o = 32; o > 0; o >>= 1)
+        v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ inline double wave_max_d(double v)
+{
+    for (int o = 32; o > 0; o >>= 1)
+        v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Kick and drift (grid-stride) that also keeps the extrema of the new
+// positions: each block writes its six partial extrema in the layout of the
+// build's AABB pass, which then only combines them (same values, so the same
+// root geometry bit for bit).
+__global__ __launch_bounds__(256) void kick_drift_aabb_kernel(
+    double* __restrict__ px, double* __restrict__ py, double* __restrict__ pz,
+    double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
+    const float* __restrict__ ax, const float* __restrict__ ay, const float* __restrict__ az,
+    size_t n, double dtk, double dtd, double vmax, double box, double* __restrict__ partial)
+{
+    __shared__ double red[6][4];
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+    double hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += stride) {
+        double x = vx[i] + static_cast<double>(ax[i]) * dtk;
+        double y = vy[i] + static_cast<double>(ay[i]) * dtk;
+        double z = vz[i] + static_cast<double>(az[i]) * dtk;
+        clamp3(x, y, z, vmax);
+        vx[i] = x;
+        vy[i] = y;
+        vz[i] = z;
+        const double nx = wrap_box(px[i] + x * dtd, box);
+        const double ny = wrap_box(py[i] + y * dtd, box);
+        const double nz = wrap_box(pz[i] + z * dtd, box);
+        px[i] = nx;
+        py[i] = ny;
+        pz[i] = nz;
+        lo[0] = fmin(lo[0], nx);
+        lo[1] = fmin(lo[1], ny);
+        lo[2] = fmin(lo[2], nz);
+        hi[0] = fmax(hi[0], nx);
+        hi[1] = fmax(hi[1], ny);
+        hi[2] = fmax(hi[2], nz);
+    }
+    const int w = threadIdx.x >> 6;
+    for (int a = 0; a < 3; ++a) {
+        const double l = wave_min_d(lo[a]);
+        const double h = wave_max_d(hi[a]);
+        if ((threadIdx.x & 63) == 0) {
+            red[a][w] = l;
+            red[3 + a][w] = h;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int a = threadIdx.x;
+        double v = red[a][0];
+        for (int k = 1; k < 4; ++k)
+            v = a < 3 ? fmin(v, red[a][k]) : fmax(v, red[a][k]);
+        partial[6 * blockIdx.x + a] = v;
     }
 }
 
@@ -262,13 +333,16 @@ void launch_kick_drift(Device& d, double dt_kick, double dt_drift, double max_sp
 {
     if (d.n == 0)
         return;
-    kick_drift_kernel<<<blocks_for(d.n), 256, 0, d.stream>>>(
+    const unsigned nb = std::min(blocks_for(d.n), static_cast<unsigned>(kAabbBlocks));
+    kick_drift_aabb_kernel<<<nb, 256, 0, d.stream>>>(
         d.pos[0], d.pos[1], d.pos[2], d.vel[0], d.vel[1], d.vel[2], d.acc[0], d.acc[1], d.acc[2],
-        d.n, dt_kick, dt_drift, max_speed, box);
+        d.n, dt_kick, dt_drift, max_speed, box, d.diag);
+    d.aabb_parts = static_cast<int>(nb);
 }
 
 void launch_wrap(Device& d, double box)
 {
+    d.aabb_parts = 0;
     if (d.n == 0 || box <= 0.0)
         return;
     wrap_kernel<<<blocks_for(d.n), 256, 0, d.stream>>>(d.pos[0], d.pos[1], d.pos[2], d.n, box);
@@ -283,6 +357,7 @@ void launch_init_ids(Device& d)
 
 void launch_upload_scatter(Device& d, int which)
 {
+    d.aabb_parts = 0;
     if (d.n == 0)
         return;
     double** dst = which == 0 ? d.pos : d.vel;
@@ -316,6 +391,7 @@ void launch_direct_fp64(Device& d, const uint32_t* targets_dev, uint32_t nt, dou
 
 void launch_energy(Device& d, double* out_dev)
 {
+    d.aabb_parts = 0;  // diag is overwritten
     const int nb = 256;
     energy_partial_kernel<<<nb, 256, 0, d.stream>>>(d.pos[0], d.pos[1], d.pos[2], d.vel[0],
                                                     d.vel[1], d.vel[2], d.mass, d.n, d.diag);
@@ -501,6 +577,7 @@ __global__ __launch_bounds__(256) void gather_u32_kernel(const uint32_t* __restr
 
 void launch_adaptive_init(Device& d, const AdaptiveParams& a, uint32_t tick)
 {
+    d.aabb_parts = 0;
     if (d.n == 0)
         return;
     adaptive_init_kernel<<<blocks_for(d.n), 256, 0, d.stream>>>(
@@ -510,6 +587,7 @@ void launch_adaptive_init(Device& d, const AdaptiveParams& a, uint32_t tick)
 
 void launch_adaptive_drift(Device& d, const AdaptiveParams& a, double span)
 {
+    d.aabb_parts = 0;
     if (d.n == 0)
         return;
     adaptive_drift_kernel<<<blocks_for(d.n), 256, 0, d.stream>>>(
@@ -519,6 +597,7 @@ void launch_adaptive_drift(Device& d, const AdaptiveParams& a, double span)
 
 void launch_adaptive_next_due(Device& d, const AdaptiveParams& a, uint32_t t, uint32_t* out)
 {
+    d.aabb_parts = 0;
     launch_fill_u32(out, 0xffffffffu, 1, d.stream);
     if (d.n == 0)
         return;
@@ -529,6 +608,7 @@ void launch_adaptive_next_due(Device& d, const AdaptiveParams& a, uint32_t t, ui
 void launch_adaptive_correct(Device& d, const AdaptiveParams& a, uint32_t tick,
                              unsigned long long* count)
 {
+    d.aabb_parts = 0;
     if (d.n == 0)
         return;
     adaptive_correct_kernel<<<reduce_blocks(d.n), 256, 0, d.stream>>>(

@@ -35,8 +35,6 @@ __device__ inline double from_order_bits(unsigned long long e)
     return __longlong_as_double(static_cast<long long>(b));
 }
 
-constexpr int kAabbBlocks = 1024;
-
 __device__ inline double wave_min(double v)
 {
     for (int o = 32; o > 0; o >>= 1)
@@ -199,9 +197,13 @@ __global__ __launch_bounds__(256) void keys_kernel(const double* __restrict__ px
 
 void launch_aabb(Device& d)
 {
-    const size_t blocks = std::min<size_t>((d.n + 255) / 256, kAabbBlocks);
-    const int nb = static_cast<int>(std::max<size_t>(blocks, 1));
-    aabb_kernel<<<nb, 256, 0, d.stream>>>(d.pos[0], d.pos[1], d.pos[2], d.n, d.diag);
+    int nb = d.aabb_parts;  // partials the drift kernel left for these positions
+    d.aabb_parts = 0;
+    if (nb <= 0) {
+        const size_t blocks = std::min<size_t>((d.n + 255) / 256, kAabbBlocks);
+        nb = static_cast<int>(std::max<size_t>(blocks, 1));
+        aabb_kernel<<<nb, 256, 0, d.stream>>>(d.pos[0], d.pos[1], d.pos[2], d.n, d.diag);
+    }
     aabb_final_kernel<<<1, 256, 0, d.stream>>>(d.diag, nb, d.aabb_bits);
     geometry_kernel<<<1, 64, 0, d.stream>>>(d.aabb_bits, d.geom);
 }
