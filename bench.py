@@ -376,8 +376,9 @@ def main(argv=None):
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print(json.dumps({"error": "octree_hip runs on one GPU; multi-GPU runs are not part of "
-                                   "this build (see DESIGN.md for how the walk would shard)"}))
+        print(json.dumps({"error": "octree_hip runs on one GPU: --gpus above 1 is not "
+                                   "supported (the walk would shard by Morton ranges of "
+                                   "targets, which is not implemented)"}))
         return 2
 
     import blitzar_amd as B
