@@ -152,41 +152,13 @@ def cpu_model():
     return "unknown"
 
 
-def zeldovich_lattice(ng, seed=3, rms_cells=0.3, box=1.0):
-    """ng^3 lattice displaced by a first-order Zel'dovich field: Gaussian
-    random density with P(k) ~ k^-2, displacement psi_k = i k delta_k / k^2,
-    scaled to an RMS displacement of rms_cells lattice cells per axis."""
-    rng = np.random.default_rng(seed)
-    white = rng.standard_normal((ng, ng, ng)).astype(np.float32)
-    dk = np.fft.rfftn(white)
-    del white
-    k = np.fft.fftfreq(ng, d=1.0 / ng).astype(np.float32)
-    kz = np.arange(ng // 2 + 1, dtype=np.float32)
-    k2 = k[:, None, None] ** 2 + k[None, :, None] ** 2 + kz[None, None, :] ** 2
-    k2[0, 0, 0] = 1.0
-    dk *= 1.0 / np.sqrt(k2)  # amplitude sqrt(P) ~ 1/k
-    dk[0, 0, 0] = 0.0
-    g = (np.arange(ng, dtype=np.float64) + 0.5) / ng * box
-    pos = np.empty((ng ** 3, 3), np.float64)
-    disp = []
-    for a, K in enumerate((k[:, None, None], k[None, :, None], kz[None, None, :])):
-        comp = np.fft.irfftn(1j * K * dk / k2, s=(ng, ng, ng), axes=(0, 1, 2))
-        disp.append(comp.astype(np.float32).reshape(-1))
-        del comp
-    scale = rms_cells * (box / ng) / np.sqrt(np.mean([np.mean(d * d) for d in disp]))
-    grid = np.meshgrid(g, g, g, indexing="ij")
-    for a in range(3):
-        pos[:, a] = np.mod(grid[a].reshape(-1) + scale * disp[a], box)
-    return pos
-
-
 def periodic_treepm(steps, ng=256):
     """Periodic TreePM box: ng^3 Zel'dovich lattice (seed 3, rms 0.3 cells),
     L = 1, rocFFT mesh ng^3, r_s = 1.25 cells, short-range cut at 4.5 r_s,
     theta 0.6, eps 0.05 cells, KDK dt 1e-3, zero initial velocities."""
     import blitzar_amd as B
     t0 = time.perf_counter()
-    pos = zeldovich_lattice(ng)
+    pos = B.zeldovich_lattice(ng)
     n = pos.shape[0]
     t_ic = time.perf_counter() - t0
     h = 1.0 / ng

@@ -88,6 +88,7 @@ EXPORTS = [
     "bh_hip_default_params", "bh_hip_version", "bh_hip_create", "bh_hip_destroy",
     "bh_hip_set_params", "bh_hip_upload", "bh_hip_download", "bh_hip_compute_accelerations",
     "bh_hip_step", "bh_hip_build_tree", "bh_hip_tree_info", "bh_hip_dump_tree",
+    "bh_hip_resolve_params", "bh_hip_get_params",
     "bh_hip_direct_forces_fp64", "bh_hip_diagnostics", "bh_hip_set_timing",
     "bh_hip_get_timings", "bh_hip_get_last_error", "bh_hip_sort_pairs",
     "bh_hip_lff_configure", "bh_hip_lff_collect", "bh_hip_lff_train", "bh_hip_lff_get_params",
@@ -136,6 +137,10 @@ def lib() -> C.CDLL:
     L.bh_hip_create.restype = R
     L.bh_hip_destroy.argtypes = [H]
     L.bh_hip_destroy.restype = None
+    L.bh_hip_resolve_params.argtypes = [C.POINTER(Params), C.POINTER(Params)]
+    L.bh_hip_resolve_params.restype = None
+    L.bh_hip_get_params.argtypes = [H, C.POINTER(Params)]
+    L.bh_hip_get_params.restype = R
     L.bh_hip_set_params.argtypes = [H, C.POINTER(Params)]
     L.bh_hip_set_params.restype = R
     L.bh_hip_upload.argtypes = [H, _p(np.float64), _opt(np.float64), _p(np.float32)]
@@ -208,3 +213,10 @@ def default_params() -> Params:
     p = Params()
     lib().bh_hip_default_params(C.byref(p))
     return p
+
+
+def resolve_params(p: Params) -> Params:
+    """The configuration a handle runs with for `p` (host only)."""
+    out = Params()
+    lib().bh_hip_resolve_params(C.byref(p), C.byref(out))
+    return out

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <random>
@@ -26,6 +27,12 @@ constexpr size_t kStackCap = 16384;
 
 struct StepEvents {
     hipEvent_t b0 = nullptr, s0 = nullptr, s1 = nullptr, b1 = nullptr, w1 = nullptr;
+};
+
+// Start and end of one kick / kick-drift launch (read at the end of the call,
+// so timing adds no host wait inside the step loop).
+struct IntegrationEvents {
+    hipEvent_t a = nullptr, b = nullptr;
 };
 
 }  // namespace
@@ -46,11 +53,24 @@ struct bh_hip_handle {
     bool rank_valid = true;   // rank (caller index -> slot) matches the store order
     bool timing = false;
     std::string last_error;
+    // A failed device allocation after creation leaves the handle unusable:
+    // every later call returns an error with this message.
+    std::string broken;
+    // Error word of the device copied (pinned) after each build, and the event
+    // that marks the copy; read while the walk queued behind the build runs.
+    int* err_host = nullptr;
+    hipEvent_t built = nullptr;
+    int err_carry = 0;      // error bits of this call kept across a node regrow
+    int steps_done = -1;    // bh_hip_step: steps completed before an error stopped it
+    // Test switch (environment BH_HIP_TEST_FAIL_NODE_GROWTH at creation): a
+    // node regrow asks for an impossible size, so the failed-allocation path
+    // runs for real.
+    bool test_fail_growth = false;
     unsigned long long* stats = nullptr;  // device: pp, pc, block-step particle steps
     std::vector<StepEvents> events;
     size_t events_used = 0;
-    double integrate_ms = 0.0;
-    hipEvent_t i0 = nullptr, i1 = nullptr;
+    std::vector<IntegrationEvents> int_events;
+    size_t int_used = 0;
     bh_hip_timings_t t{};
     std::vector<void*> allocations;
     // learned far-field correction
@@ -189,6 +209,10 @@ bool alloc_nodes(bh_hip_handle* h, size_t cap)
     dfree(h, d.gscan);
     dfree(h, d.gscan_blocks);
     d.nprefix = nullptr;
+    d.nstart = d.ncount = d.nchild = d.nfirst = d.scan_tmp = d.bnd_tmp = d.big_list = nullptr;
+    d.scan_blocks = d.nparent = d.gscan = d.gscan_blocks = nullptr;
+    d.nlevel = nullptr;
+    d.wnode = nullptr;
     d.node_cap = cap;
     const size_t scan_n = std::min(d.n, cap);
     return dalloc(h, &d.nprefix, cap, "node prefix") && dalloc(h, &d.nstart, cap, "node start") &&
@@ -246,9 +270,14 @@ bool alloc_all(bh_hip_handle* h, uint64_t n)
 
 bool zero_err(bh_hip_handle* h)
 {
+    h->err_carry = 0;
     bh::launch_fill_u32(reinterpret_cast<uint32_t*>(h->d.err), 0u, 1, h->d.stream);
     return check(h, hipGetLastError(), "error flag reset");
 }
+
+// Device errors that no retry repairs (a node-capacity overflow is repaired
+// inside the force evaluation, see check_build).
+constexpr int kFatalBits = bh::kErrStackOverflow | bh::kErrLeafTooLarge | bh::kErrScan;
 
 // Synchronise and turn device error bits into a message.
 bh_hip_result_t finish(bh_hip_handle* h, const char* what)
@@ -263,23 +292,79 @@ bh_hip_result_t finish(bh_hip_handle* h, const char* what)
     int flags = 0;
     if (!check(h, hipMemcpy(&flags, h->d.err, sizeof(int), hipMemcpyDeviceToHost), "error read"))
         return BH_HIP_INTERNAL_ERROR;
+    flags |= h->err_carry;
+    h->err_carry = 0;
     if (flags != 0) {
         std::string m = std::string("[octree_hip] ") + what + ":";
         if (flags & bh::kErrNodeOverflow)
-            m += " node capacity exceeded (capacity grown; results of this call degraded);";
+            m += " node capacity exceeded (tree truncated);";
         if (flags & bh::kErrStackOverflow)
             m += " traversal stack overflow;";
         if (flags & bh::kErrLeafTooLarge)
             m += " leaf larger than 2^23 particles;";
         if (flags & bh::kErrScan)
             m += " device scan did not complete (results of this call invalid);";
+        if (h->steps_done >= 0)
+            m += " stopped after " + std::to_string(h->steps_done) +
+                 " completed steps (the state holds them and part of the next)";
         h->last_error = m;
-        if (flags & bh::kErrNodeOverflow)
-            alloc_nodes(h, h->d.node_cap * 2);
         return BH_HIP_INTERNAL_ERROR;
     }
     return BH_HIP_OK;
 }
+
+bool usable(bh_hip_handle* h)
+{
+    if (h->broken.empty())
+        return true;
+    h->last_error = h->broken;
+    return false;
+}
+
+// Double the node capacity. A failed allocation leaves the handle unusable
+// (alloc_nodes has released the node arrays and nulled their pointers).
+bool grow_nodes(bh_hip_handle* h)
+{
+    const size_t cap = h->d.node_cap;
+    h->have_tree = false;
+    const size_t want = h->test_fail_growth ? (static_cast<size_t>(1) << 46) : cap * 2;
+    if (alloc_nodes(h, want))
+        return true;
+    h->d.node_cap = 0;
+    h->broken = "[octree_hip] handle unusable: growing the node arrays to " +
+                std::to_string(want) + " nodes failed (" + h->last_error + ")";
+    h->last_error = h->broken;
+    return false;
+}
+
+enum class Built { kOk, kRetry, kFatal };
+
+// Called once the work that uses a build (the walk) has been queued behind
+// it: waits for the build alone (the walk keeps the GPU busy meanwhile) and
+// reads the error word the build left. On a node-capacity overflow the walk
+// ran on a truncated tree: the call waits for it, keeps the error bits raised
+// before this build, grows the capacity and asks the caller to queue the
+// build and its walk again (positions are unchanged, so the repeat is the
+// evaluation a larger capacity would have given).
+Built check_build(bh_hip_handle* h)
+{
+    if (!check(h, hipEventSynchronize(h->built), "build"))
+        return Built::kFatal;
+    const int bits = *h->err_host;
+    if (!(bits & bh::kErrNodeOverflow))
+        return Built::kOk;
+    if (!check(h, hipStreamSynchronize(h->d.stream), "build"))
+        return Built::kFatal;
+    h->err_carry |= bits & ~bh::kErrNodeOverflow;
+    bh::launch_fill_u32(reinterpret_cast<uint32_t*>(h->d.err), 0u, 1, h->d.stream);
+    if (!grow_nodes(h))
+        return Built::kFatal;
+    h->amag_valid = false;  // the truncated walk wrote |a|
+    return Built::kRetry;
+}
+
+// Bits of the device error word at the last build (and before it).
+int bits_at_build(const bh_hip_handle* h) { return *h->err_host | h->err_carry; }
 
 StepEvents* next_events(bh_hip_handle* h)
 {
@@ -294,6 +379,23 @@ StepEvents* next_events(bh_hip_handle* h)
         h->events.push_back(e);
     }
     return &h->events[h->events_used++];
+}
+
+IntegrationEvents* next_integration_events(bh_hip_handle* h)
+{
+    if (!h->timing)
+        return nullptr;
+    if (h->int_used == h->int_events.size()) {
+        IntegrationEvents e;
+        if (hipEventCreate(&e.a) != hipSuccess)
+            return nullptr;
+        if (hipEventCreate(&e.b) != hipSuccess) {
+            (void)hipEventDestroy(e.a);
+            return nullptr;
+        }
+        h->int_events.push_back(e);
+    }
+    return &h->int_events[h->int_used++];
 }
 
 void rec(hipEvent_t e, hipStream_t s)
@@ -321,7 +423,25 @@ void queue_build(bh_hip_handle* h, StepEvents* ev)
         bh::launch_groups(d);
     }
     rec(ev ? ev->b1 : nullptr, d.stream);
+    (void)hipMemcpyAsync(h->err_host, d.err, sizeof(int), hipMemcpyDeviceToHost, d.stream);
+    (void)hipEventRecord(h->built, d.stream);
     h->have_tree = true;
+}
+
+// A build with nothing queued behind it (topology dumps), regrown and
+// repeated on a node-capacity overflow.
+bool build_checked(bh_hip_handle* h, StepEvents* ev)
+{
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        queue_build(h, ev);
+        const Built b = check_build(h);
+        if (b == Built::kOk)
+            return true;
+        if (b == Built::kFatal)
+            return false;
+    }
+    h->last_error = "[octree_hip] build: node capacity still exceeded after 8 doublings";
+    return false;
 }
 
 void ensure_rank(bh_hip_handle* h)
@@ -388,16 +508,15 @@ const char* lff_incompatible(const bh_hip_params_t& p)
         return "learned far-field correction is not available with the TreePM short-range walk";
     if (p.key_mode != 0)
         return "learned far-field correction needs the 63-bit keys";
+    if (p.criterion == BH_HIP_CRITERION_RELATIVE)
+        return "learned far-field correction runs the geometric criteria (com, bounds, extent), "
+               "not the relative criterion";
     return nullptr;
 }
 
-// Queue build + walk (a force evaluation); with the learned correction
-// enabled the walk runs at the correction's opening angle, records the near
-// nodes of each group and the MLP pass adds its corrections.
-void queue_force(bh_hip_handle* h)
+// The walk(s) of one force evaluation, queued behind a build.
+void queue_walks(bh_hip_handle* h)
 {
-    StepEvents* ev = next_events(h);
-    queue_build(h, ev);
     if (lff_ready(h)) {
         bh::WalkParams wp = walk_params(h->p);
         wp.theta2 = h->lff_cfg.theta * h->lff_cfg.theta;
@@ -415,15 +534,38 @@ void queue_force(bh_hip_handle* h)
     } else {
         launch_force_walk(h, walk_params(h->p));
     }
+}
+
+// Queue build + walk (a force evaluation); with the learned correction
+// enabled the walk runs at the correction's opening angle, records the near
+// nodes of each group and the MLP pass adds its corrections. Returns false
+// on an error that stops the call (message in last_error).
+bool queue_force(bh_hip_handle* h)
+{
+    StepEvents* ev = next_events(h);
+    for (int attempt = 0;; ++attempt) {
+        queue_build(h, ev);
+        queue_walks(h);
+        const Built b = check_build(h);
+        if (b == Built::kOk)
+            break;
+        if (b == Built::kFatal)
+            return false;
+        if (attempt == 7) {
+            h->last_error = "[octree_hip] node capacity still exceeded after 8 doublings";
+            return false;
+        }
+    }
     rec(ev ? ev->w1 : nullptr, h->d.stream);
     h->primed = true;
+    return true;
 }
 
 
 void begin_call(bh_hip_handle* h)
 {
     h->events_used = 0;
-    h->integrate_ms = 0.0;
+    h->int_used = 0;
     bh::launch_fill_u32(reinterpret_cast<uint32_t*>(h->stats), 0u, 12, h->d.stream);
 }
 
@@ -442,7 +584,11 @@ void end_call(bh_hip_handle* h)
             if (hipEventElapsedTime(&c, e.b1, e.w1) == hipSuccess)
                 t.walk_ms += c;
         }
-        t.integrate_ms = h->integrate_ms;
+        for (size_t k = 0; k < h->int_used; ++k) {
+            float a = 0.0f;
+            if (hipEventElapsedTime(&a, h->int_events[k].a, h->int_events[k].b) == hipSuccess)
+                t.integrate_ms += a;
+        }
     }
     unsigned long long st[6] = {0, 0, 0, 0, 0, 0};
     if (hipMemcpy(st, h->stats, sizeof(st), hipMemcpyDeviceToHost) == hipSuccess) {
@@ -509,6 +655,25 @@ bool alloc_adaptive(bh_hip_handle* h)
     return true;
 }
 
+// Build + walk of one block-step force evaluation, regrown and repeated on a
+// node-capacity overflow.
+bool build_and_walk(bh_hip_handle* h, StepEvents* ev, const bh::WalkParams& wp)
+{
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        queue_build(h, ev);
+        launch_force_walk(h, wp);
+        const Built b = check_build(h);
+        if (b == Built::kOk) {
+            rec(ev ? ev->w1 : nullptr, h->d.stream);
+            return true;
+        }
+        if (b == Built::kFatal)
+            return false;
+    }
+    h->last_error = "[octree_hip] node capacity still exceeded after 8 doublings";
+    return false;
+}
+
 bh_hip_result_t step_adaptive(bh_hip_handle* h, double dt, int32_t nsteps)
 {
     if (lff_ready(h)) {
@@ -521,14 +686,19 @@ bh_hip_result_t step_adaptive(bh_hip_handle* h, double dt, int32_t nsteps)
     bh::Device& d = h->d;
     const bh::AdaptiveParams a = adaptive_params(h, dt);
     const bh::WalkParams wp0 = walk_params(h->p);
+    auto fail = [&]() {
+        (void)hipStreamSynchronize(d.stream);
+        h->ad_primed = false;
+        h->primed = false;
+        end_call(h);
+        return BH_HIP_INTERNAL_ERROR;
+    };
     if (!h->ad_primed || std::fabs(h->ad_quantum - a.quantum) > 1e-12 * a.quantum) {
-        // ticks restart at 0 so that every cadence 2^(L - level) divides the
+        // ticks start at 0 so that every cadence 2^(L - level) divides the
         // start of each global step
         h->ad_tick = 0;
-        StepEvents* ev = next_events(h);
-        queue_build(h, ev);
-        launch_force_walk(h, wp0);
-        rec(ev ? ev->w1 : nullptr, d.stream);
+        if (!build_and_walk(h, next_events(h), wp0))
+            return fail();
         bh::launch_adaptive_init(d, a, 0u);
         h->ad_primed = true;
         h->ad_quantum = a.quantum;
@@ -539,7 +709,9 @@ bh_hip_result_t step_adaptive(bh_hip_handle* h, double dt, int32_t nsteps)
     // single-slice predictors one predictor over k q).
     uint32_t* due = reinterpret_cast<uint32_t*>(d.diag);
     const uint64_t slices = 1ull << a.levels;
-    for (int k = 0; k < nsteps; ++k) {
+    h->steps_done = 0;
+    bool stop = false;
+    for (int k = 0; k < nsteps && !stop; ++k) {
         const uint64_t end = h->ad_tick + slices;
         while (h->ad_tick < end) {
             const uint32_t t0 = static_cast<uint32_t>(h->ad_tick);
@@ -548,27 +720,38 @@ bh_hip_result_t step_adaptive(bh_hip_handle* h, double dt, int32_t nsteps)
             if (!check(h, hipMemcpyAsync(&next, due, sizeof(next), hipMemcpyDeviceToHost,
                                          d.stream),
                        "adaptive schedule") ||
-                !check(h, hipStreamSynchronize(d.stream), "adaptive schedule")) {
-                end_call(h);
-                return BH_HIP_INTERNAL_ERROR;
-            }
+                !check(h, hipStreamSynchronize(d.stream), "adaptive schedule"))
+                return fail();
             const uint64_t t = std::min<uint64_t>(end, h->ad_tick + (next - t0));
             bh::launch_adaptive_drift(d, a, static_cast<double>(t - h->ad_tick) * a.quantum);
             h->ad_tick = t;
             const uint32_t tt = static_cast<uint32_t>(t);
-            StepEvents* ev = next_events(h);
-            queue_build(h, ev);
             bh::WalkParams wp = wp0;
             wp.alevel = d.alevel;
             wp.atick = tt;
             wp.alevels = a.levels;
-            launch_force_walk(h, wp);
-            rec(ev ? ev->w1 : nullptr, d.stream);
+            if (!build_and_walk(h, next_events(h), wp))
+                return fail();
             bh::launch_adaptive_correct(d, a, tt, h->stats + 2);
+            if (bits_at_build(h) & kFatalBits) {
+                stop = true;
+                break;
+            }
         }
+        if (stop)
+            break;
+        // Every particle was corrected at the last tick of the global step
+        // (each cadence divides 2^L), so every last-force tick equals `end`:
+        // ticks restart at 0 and stay below 2^L + 1 however long the run.
+        bh::launch_fill_u32(d.atick, 0u, d.n, d.stream);
+        h->ad_tick = 0;
+        h->steps_done = k + 1;
     }
     h->primed = false;
     const bh_hip_result_t r = finish(h, "step (block time steps)");
+    h->steps_done = -1;
+    if (r != BH_HIP_OK)
+        h->ad_primed = false;
     end_call(h);
     return r;
 }
@@ -610,7 +793,21 @@ void bh_hip_default_params(bh_hip_params_t* out)
     out->relative_alpha = kDefaultRelativeAlpha;
 }
 
-const char* bh_hip_version(void) { return "octree_hip 0.1 (gfx950)"; }
+const char* bh_hip_version(void) { return "octree_hip 0.2 (gfx950)"; }
+
+void bh_hip_resolve_params(const bh_hip_params_t* in, bh_hip_params_t* out)
+{
+    if (in && out)
+        *out = resolve(*in);
+}
+
+bh_hip_result_t bh_hip_get_params(bh_hip_handle_t* h, bh_hip_params_t* out)
+{
+    if (!h || !out)
+        return BH_HIP_INVALID_ARGUMENT;
+    *out = h->p;
+    return BH_HIP_OK;
+}
 
 bh_hip_result_t bh_hip_create(const bh_hip_params_t* params, uint64_t n, bh_hip_handle_t** out,
                               char* err, size_t err_cap)
@@ -636,6 +833,7 @@ bh_hip_result_t bh_hip_create(const bh_hip_params_t* params, uint64_t n, bh_hip_
         return BH_HIP_INTERNAL_ERROR;
     }
     h->p = resolve(*params);
+    h->test_fail_growth = std::getenv("BH_HIP_TEST_FAIL_NODE_GROWTH") != nullptr;
     int dev = 0;
     hipDeviceProp_t prop{};
     if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
@@ -644,7 +842,12 @@ bh_hip_result_t bh_hip_create(const bh_hip_params_t* params, uint64_t n, bh_hip_
     h->leaf_depth = reference_leaf_depth(n, h->p.leaf_capacity);
     bool ok = check(h, hipStreamCreateWithFlags(&h->d.stream, hipStreamNonBlocking), "stream");
     ok = ok && alloc_all(h, n);
-    ok = ok && check(h, hipEventCreate(&h->i0), "event") && check(h, hipEventCreate(&h->i1), "event");
+    ok = ok && check(h, hipEventCreateWithFlags(&h->built, hipEventDisableTiming), "event") &&
+         check(h, hipHostMalloc(reinterpret_cast<void**>(&h->err_host), sizeof(int),
+                                hipHostMallocDefault),
+               "pinned error word");
+    if (ok)
+        *h->err_host = 0;
     if (ok && h->p.pm_mesh > 0 && !bh::pm_setup(h->pm, h->p.pm_mesh, h->d.stream)) {
         h->last_error = "bh_hip_create: particle-mesh setup (rocFFT plans, meshes) failed";
         ok = false;
@@ -677,10 +880,14 @@ void bh_hip_destroy(bh_hip_handle_t* h)
             if (x)
                 (void)hipEventDestroy(x);
     }
-    if (h->i0)
-        (void)hipEventDestroy(h->i0);
-    if (h->i1)
-        (void)hipEventDestroy(h->i1);
+    for (auto& e : h->int_events) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    if (h->built)
+        (void)hipEventDestroy(h->built);
+    if (h->err_host)
+        (void)hipHostFree(h->err_host);
     if (h->d.stream)
         (void)hipStreamDestroy(h->d.stream);
     delete h;
@@ -690,6 +897,8 @@ bh_hip_result_t bh_hip_set_params(bh_hip_handle_t* h, const bh_hip_params_t* par
 {
     if (!h || !params)
         return BH_HIP_INVALID_ARGUMENT;
+    if (!usable(h))
+        return BH_HIP_INTERNAL_ERROR;
     const bh_hip_params_t old = h->p;
     const bh_hip_params_t next = resolve(*params);
     if (const char* why = invalid_combination(next)) {
@@ -730,6 +939,8 @@ bh_hip_result_t bh_hip_upload(bh_hip_handle_t* h, const double* pos3, const doub
 {
     if (!h || !pos3 || !mass)
         return BH_HIP_INVALID_ARGUMENT;
+    if (!usable(h))
+        return BH_HIP_INTERNAL_ERROR;
     bh::Device& d = h->d;
     // The store keeps its current order; rank maps caller index -> slot.
     std::vector<float> m_sorted;
@@ -779,6 +990,8 @@ bh_hip_result_t bh_hip_download(bh_hip_handle_t* h, double* pos3, double* vel3)
 {
     if (!h)
         return BH_HIP_INVALID_ARGUMENT;
+    if (!usable(h))
+        return BH_HIP_INTERNAL_ERROR;
     bh::Device& d = h->d;
     ensure_rank(h);
     for (int which = 0; which < 2; ++which) {
@@ -799,10 +1012,14 @@ bh_hip_result_t bh_hip_build_tree(bh_hip_handle_t* h)
 {
     if (!h)
         return BH_HIP_INVALID_ARGUMENT;
-    if (!zero_err(h))
+    if (!usable(h) || !zero_err(h))
         return BH_HIP_INTERNAL_ERROR;
     begin_call(h);
-    queue_build(h, next_events(h));
+    if (!build_checked(h, next_events(h))) {
+        (void)hipStreamSynchronize(h->d.stream);
+        end_call(h);
+        return BH_HIP_INTERNAL_ERROR;
+    }
     const bh_hip_result_t r = finish(h, "build_tree");
     end_call(h);
     return r;
@@ -816,10 +1033,15 @@ bh_hip_result_t bh_hip_compute_accelerations(bh_hip_handle_t* h, float* acc3)
         h->last_error = "compute_accelerations: key_mode REFERENCE30 builds topology only";
         return BH_HIP_INVALID_ARGUMENT;
     }
-    if (!zero_err(h))
+    if (!usable(h) || !zero_err(h))
         return BH_HIP_INTERNAL_ERROR;
     begin_call(h);
-    queue_force(h);
+    if (!queue_force(h)) {
+        (void)hipStreamSynchronize(h->d.stream);
+        h->primed = false;
+        end_call(h);
+        return BH_HIP_INTERNAL_ERROR;
+    }
     bh_hip_result_t r = finish(h, "compute_accelerations");
     end_call(h);
     if (r != BH_HIP_OK || !acc3)
@@ -846,51 +1068,64 @@ bh_hip_result_t bh_hip_step(bh_hip_handle_t* h, double dt, int32_t nsteps)
         h->last_error = "step: key_mode REFERENCE30 builds topology only";
         return BH_HIP_INVALID_ARGUMENT;
     }
-    if (!zero_err(h))
+    if (!usable(h) || !zero_err(h))
         return BH_HIP_INTERNAL_ERROR;
     if (h->p.adaptive_max_level > 0)
         return step_adaptive(h, dt, nsteps);
     begin_call(h);
     bh::Device& d = h->d;
     const double vmax = h->p.max_speed;
-    float int_ms = 0.0f;
-    auto time_integration = [&]() {
-        if (h->timing && hipEventSynchronize(h->i1) == hipSuccess &&
-            hipEventElapsedTime(&int_ms, h->i0, h->i1) == hipSuccess)
-            h->integrate_ms += int_ms;
+    auto fail = [&]() {
+        (void)hipStreamSynchronize(d.stream);
+        h->primed = false;
+        h->steps_done = -1;
+        end_call(h);
+        return BH_HIP_INTERNAL_ERROR;
     };
     // Leapfrog: the closing half kick of a step and the opening half kick of
     // the next use the same acceleration, so without a velocity clamp they
     // are applied as one kick of dt (v + a dt instead of (v + a dt/2) + a dt/2,
     // which differ by rounding only). The last step of the call keeps its
     // closing half kick: velocities leave the call at whole steps.
+    // A device error that no retry repairs (read at each build) stops the
+    // loop; the message then says how many steps completed.
     bool half_kick_due = false;
+    h->steps_done = 0;
     for (int k = 0; k < nsteps; ++k) {
         if (h->p.integrator == BH_HIP_INTEGRATOR_EULER) {
-            queue_force(h);
-            rec(h->timing ? h->i0 : nullptr, d.stream);
+            if (!queue_force(h))
+                return fail();
+            if (bits_at_build(h) & kFatalBits)
+                break;
+            IntegrationEvents* ie = next_integration_events(h);
+            rec(ie ? ie->a : nullptr, d.stream);
             bh::launch_kick_drift(d, dt, dt, vmax, h->p.periodic_box);
-            rec(h->timing ? h->i1 : nullptr, d.stream);
+            rec(ie ? ie->b : nullptr, d.stream);
             h->primed = false;
-            time_integration();
         } else {
-            if (!h->primed)
-                queue_force(h);
-            rec(h->timing ? h->i0 : nullptr, d.stream);
+            if (!h->primed && !queue_force(h))
+                return fail();
+            IntegrationEvents* ie = next_integration_events(h);
+            rec(ie ? ie->a : nullptr, d.stream);
             bh::launch_kick_drift(d, half_kick_due ? dt : 0.5 * dt, dt, vmax, h->p.periodic_box);
-            rec(h->timing ? h->i1 : nullptr, d.stream);
-            time_integration();
-            queue_force(h);
-            half_kick_due = vmax <= 0.0 && k + 1 < nsteps;
+            rec(ie ? ie->b : nullptr, d.stream);
+            if (!queue_force(h))
+                return fail();
+            const bool fatal = (bits_at_build(h) & kFatalBits) != 0;
+            half_kick_due = vmax <= 0.0 && k + 1 < nsteps && !fatal;
             if (!half_kick_due) {
-                rec(h->timing ? h->i0 : nullptr, d.stream);
+                ie = next_integration_events(h);
+                rec(ie ? ie->a : nullptr, d.stream);
                 bh::launch_kick(d, 0.5 * dt, vmax);
-                rec(h->timing ? h->i1 : nullptr, d.stream);
-                time_integration();
+                rec(ie ? ie->b : nullptr, d.stream);
             }
+            if (fatal)
+                break;
         }
+        h->steps_done = k + 1;
     }
     const bh_hip_result_t r = finish(h, "step");
+    h->steps_done = -1;
     end_call(h);
     return r;
 }
@@ -900,6 +1135,8 @@ bh_hip_result_t bh_hip_tree_info(bh_hip_handle_t* h, uint64_t* node_count, int32
 {
     if (!h)
         return BH_HIP_INVALID_ARGUMENT;
+    if (!usable(h))
+        return BH_HIP_INTERNAL_ERROR;
     int info[2 * bh::kLevels + 4];
     if (!check(h, hipStreamSynchronize(h->d.stream), "tree_info") ||
         !check(h, hipMemcpy(info, h->d.level_info, sizeof(info), hipMemcpyDeviceToHost),
@@ -922,6 +1159,8 @@ bh_hip_result_t bh_hip_dump_tree(bh_hip_handle_t* h, uint64_t* sorted_keys, uint
 {
     if (!h)
         return BH_HIP_INVALID_ARGUMENT;
+    if (!usable(h))
+        return BH_HIP_INTERNAL_ERROR;
     if (!h->have_tree) {
         h->last_error = "dump_tree: no tree built for the current positions";
         return BH_HIP_NOT_READY;
@@ -982,6 +1221,8 @@ bh_hip_result_t bh_hip_direct_forces_fp64(bh_hip_handle_t* h, const uint32_t* ta
 {
     if (!h || (!targets && nt) || (!acc3 && nt))
         return BH_HIP_INVALID_ARGUMENT;
+    if (!usable(h))
+        return BH_HIP_INTERNAL_ERROR;
     if (nt == 0)
         return BH_HIP_OK;
     for (uint32_t i = 0; i < nt; ++i)
@@ -1010,6 +1251,8 @@ bh_hip_result_t bh_hip_diagnostics(bh_hip_handle_t* h, uint32_t sample_limit, do
 {
     if (!h || !out8)
         return BH_HIP_INVALID_ARGUMENT;
+    if (!usable(h))
+        return BH_HIP_INTERNAL_ERROR;
     bh::Device& d = h->d;
     double* dres = nullptr;
     uint32_t* dslots = nullptr;
@@ -1193,6 +1436,8 @@ bh_hip_result_t bh_hip_lff_configure(bh_hip_handle_t* h, const bh_hip_lff_config
     if (!h || !cfg || !(cfg->theta > 0.0f) || !(cfg->band > 0.0f) || cfg->list_cap < 0 ||
         cfg->min_count < 0)
         return BH_HIP_INVALID_ARGUMENT;
+    if (!usable(h))
+        return BH_HIP_INTERNAL_ERROR;
     if (cfg->enabled) {
         if (const char* why = lff_incompatible(h->p)) {
             h->last_error = std::string("lff_configure: ") + why;
@@ -1213,6 +1458,8 @@ bh_hip_result_t bh_hip_lff_collect(bh_hip_handle_t* h, int32_t group_stride, uin
 {
     if (!h || group_stride < 1 || max_tuples == 0 || max_tuples > 0xffffffffull)
         return BH_HIP_INVALID_ARGUMENT;
+    if (!usable(h))
+        return BH_HIP_INTERNAL_ERROR;
     if (const char* why = lff_incompatible(h->p)) {
         h->last_error = std::string("lff_collect: ") + why;
         return BH_HIP_INVALID_ARGUMENT;
@@ -1230,10 +1477,20 @@ bh_hip_result_t bh_hip_lff_collect(bh_hip_handle_t* h, int32_t group_stride, uin
     if (!zero_err(h))
         return BH_HIP_INTERNAL_ERROR;
     begin_call(h);
-    queue_build(h, nullptr);
     bh::WalkParams wp = walk_params(h->p);
     wp.theta2 = h->lff_cfg.theta * h->lff_cfg.theta;
-    bh::launch_walk_record(h->d, wp, h->stats, L, h->lff_cfg.band, h->lff_cfg.min_count);
+    for (int attempt = 0;; ++attempt) {
+        queue_build(h, nullptr);
+        bh::launch_walk_record(h->d, wp, h->stats, L, h->lff_cfg.band, h->lff_cfg.min_count);
+        const Built b = check_build(h);
+        if (b == Built::kOk)
+            break;
+        if (b == Built::kFatal || attempt == 7) {
+            (void)hipStreamSynchronize(h->d.stream);
+            end_call(h);
+            return BH_HIP_INTERNAL_ERROR;
+        }
+    }
     // pass 1: count the candidate pairs
     bh::launch_fill_u32(L.tuple_counter + 1, 0u, 1, h->d.stream);
     bh::launch_lff_tuples(h->d, L, group_stride, h->lff_cfg.band, wp.eps2, h->lff_cfg.min_count,
@@ -1271,6 +1528,8 @@ bh_hip_result_t bh_hip_lff_train(bh_hip_handle_t* h, int32_t epochs, int32_t bat
 {
     if (!h || epochs < 1 || batch < 64 || !(lr > 0.0f))
         return BH_HIP_INVALID_ARGUMENT;
+    if (!usable(h))
+        return BH_HIP_INTERNAL_ERROR;
     bh::LffState& L = h->lff;
     if (!L.X || !L.params) {
         h->last_error = "lff_train: no training tuples (call bh_hip_lff_collect first)";
@@ -1473,6 +1732,8 @@ extern "C" bh_hip_result_t bh_hip_adaptive_levels(bh_hip_handle_t* h, int32_t* l
 {
     if (!h || !levels)
         return BH_HIP_INVALID_ARGUMENT;
+    if (!usable(h))
+        return BH_HIP_INTERNAL_ERROR;
     bh::Device& d = h->d;
     if (!d.alevel) {
         std::memset(levels, 0, d.n * sizeof(int32_t));
@@ -1493,6 +1754,8 @@ extern "C" bh_hip_result_t bh_hip_pm_accelerations(bh_hip_handle_t* h, float* ac
 {
     if (!h || !acc3)
         return BH_HIP_INVALID_ARGUMENT;
+    if (!usable(h))
+        return BH_HIP_INTERNAL_ERROR;
     if (h->pm.N == 0) {
         h->last_error = "pm_accelerations: no particle mesh (pm_mesh = 0)";
         return BH_HIP_NOT_READY;

@@ -268,6 +268,13 @@ void launch_keys(Device& d, int key_mode);
 // bits [0, key_bits). Result lands back in d.keys / d.vals.
 void launch_radix_sort(Device& d, size_t n, int key_bits);
 size_t radix_hist_words(size_t n);
+// The same sort on explicit buffers: (keys, vals) in, (keys_alt, vals_alt) the
+// second buffer pair, hist_ws radix_hist_words(n) words whose scan part
+// starts zeroed once. Returns true when the result ends in the alternate
+// buffers (an odd number of passes). Nothing is allocated.
+int radix_sort_passes(int key_bits);
+bool radix_sort_buffers(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt,
+                        uint32_t* hist_ws, size_t n, int key_bits, int* err, hipStream_t stream);
 // Exclusive scan of `data` in place (one launch). The element count is *n_dev
 // when n_dev is non-null (then max_n bounds it), else n_host. The total is
 // written to total_out / total_out_i when non-null. `block_sums` is the scan's

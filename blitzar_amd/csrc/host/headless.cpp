@@ -177,8 +177,22 @@ int main(int argc, char** argv)
         problems.push_back("dt must be positive");
     for (const auto& p : problems)
         std::cout << "[validate] error: " << p << "\n";
-    if (a.command == "inspect")
+    if (a.command == "inspect") {
+        // the values the solver would run with (theta clamp of
+        // SrvFormatAndTheta.cpp:54-69, then resolveForceLawPolicy)
+        bh_hip_params_t in;
+        bh_hip_default_params(&in);
+        in.theta = bhh::resolve_theta(cfg);
+        in.softening = cfg.softening;
+        in.min_softening = cfg.min_softening;
+        in.min_distance2 = cfg.min_distance2;
+        in.min_theta = cfg.min_theta;
+        bh_hip_params_t r;
+        bh_hip_resolve_params(&in, &r);
+        std::cout << "[headless] resolved theta=" << r.theta << " softening=" << r.softening
+                  << " min_theta=" << r.min_theta << " min_distance2=" << r.min_distance2 << "\n";
         return 0;
+    }
     if (a.command == "validate")
         return problems.empty() ? 0 : 4;
     if (!problems.empty())

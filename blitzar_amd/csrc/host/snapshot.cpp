@@ -12,6 +12,21 @@ namespace {
 
 constexpr char kMagic[8] = {'N', 'B', 'S', 'I', 'M', 'B', 'I', 'N'};
 constexpr uint32_t kVersion = 1;
+// Particle counts a snapshot may declare: at least 2, as the reference's
+// import check (engine/server/simulation/runtime/SrvHelpers.cpp:207-210), and
+// at most what one solver handle takes (2^31 - 1). The reference's 2M import
+// cap (SrvConfigurationInternal.hpp:20) is not kept: 8M states are read.
+constexpr uint64_t kMinCount = 2;
+constexpr uint64_t kMaxCount = 0x7fffffffull;
+// Shortest xyz row: "P 0 0 0 1" and a newline.
+constexpr uint64_t kMinXyzRow = 10;
+
+uint64_t file_size(const std::string& path)
+{
+    std::error_code ec;
+    const auto n = std::filesystem::file_size(path, ec);
+    return ec ? 0 : static_cast<uint64_t>(n);
+}
 
 struct Header {
     char magic[8];
@@ -44,6 +59,14 @@ bool read_snapshot(const std::string& path, const std::string& format, State& s,
             err = "not an NBSIMBIN v1 file: " + path;
             return false;
         }
+        // The declared count is checked against the bounds and the file size
+        // before anything is allocated.
+        const uint64_t need = sizeof(Header) + static_cast<uint64_t>(h.count) * sizeof(Record);
+        if (h.count < kMinCount || h.count > kMaxCount || need > file_size(path)) {
+            err = "NBSIMBIN header declares " + std::to_string(h.count) +
+                  " particles, outside [2, 2^31 - 1] or beyond the file size: " + path;
+            return false;
+        }
         std::vector<Record> r(h.count);
         if (h.count && !in.read(reinterpret_cast<char*>(r.data()),
                                 static_cast<std::streamsize>(r.size() * sizeof(Record)))) {
@@ -72,6 +95,11 @@ bool read_snapshot(const std::string& path, const std::string& format, State& s,
         std::string line;
         if (!(in >> n) || !std::getline(in, line) || !std::getline(in, line)) {
             err = "bad xyz header: " + path;
+            return false;
+        }
+        if (n < kMinCount || n > kMaxCount || n > file_size(path) / kMinXyzRow) {
+            err = "xyz header declares " + std::to_string(n) +
+                  " particles, outside [2, 2^31 - 1] or beyond the file size: " + path;
             return false;
         }
         s.pos.assign(3 * n, 0.0);

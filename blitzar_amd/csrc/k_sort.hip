@@ -273,32 +273,43 @@ size_t radix_hist_words(size_t n)
     return 256 * tiles + scan_block_sums_words(256 * tiles);
 }
 
-void launch_radix_sort(Device& d, size_t n, int key_bits)
+int radix_sort_passes(int key_bits) { return (key_bits + 7) / 8; }
+
+bool radix_sort_buffers(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt,
+                        uint32_t* hist_ws, size_t n, int key_bits, int* err, hipStream_t stream)
 {
+    const int passes = radix_sort_passes(key_bits);
     if (n <= 1)
-        return;
+        return false;
     const uint32_t ntiles = static_cast<uint32_t>((n + kTile - 1) / kTile);
     const size_t hn = static_cast<size_t>(ntiles) * 256;
-    uint32_t* hist = d.sort_hist;
-    uint32_t* sums = d.sort_hist + hn;
-    unsigned long long* kin = reinterpret_cast<unsigned long long*>(d.keys);
-    unsigned long long* kout = reinterpret_cast<unsigned long long*>(d.keys_b);
-    uint32_t* vin = d.vals;
-    uint32_t* vout = d.vals_b;
-    const int passes = (key_bits + 7) / 8;
+    uint32_t* hist = hist_ws;
+    uint32_t* sums = hist_ws + hn;
+    unsigned long long* kin = reinterpret_cast<unsigned long long*>(keys);
+    unsigned long long* kout = reinterpret_cast<unsigned long long*>(keys_alt);
+    uint32_t* vin = vals;
+    uint32_t* vout = vals_alt;
     for (int p = 0; p < passes; ++p) {
         const int shift = 8 * p;
         // The last digit covers only the remaining key bits.
         const int bits = key_bits - shift < 8 ? key_bits - shift : 8;
         const uint32_t dmask = (1u << bits) - 1u;
-        hist_kernel<<<ntiles, kBlock, 0, d.stream>>>(kin, n, shift, dmask, hist, ntiles);
-        launch_scan_exclusive(hist, hn, nullptr, hn, sums, nullptr, nullptr, d.err, d.stream);
-        scatter_kernel<<<ntiles, kBlock, 0, d.stream>>>(kin, vin, kout, vout, n, shift, dmask,
-                                                        hist, ntiles);
+        hist_kernel<<<ntiles, kBlock, 0, stream>>>(kin, n, shift, dmask, hist, ntiles);
+        launch_scan_exclusive(hist, hn, nullptr, hn, sums, nullptr, nullptr, err, stream);
+        scatter_kernel<<<ntiles, kBlock, 0, stream>>>(kin, vin, kout, vout, n, shift, dmask,
+                                                      hist, ntiles);
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
-    if (passes & 1) {
+    return (passes & 1) != 0;
+}
+
+void launch_radix_sort(Device& d, size_t n, int key_bits)
+{
+    if (n <= 1)
+        return;
+    if (radix_sort_buffers(d.keys, d.keys_b, d.vals, d.vals_b, d.sort_hist, n, key_bits, d.err,
+                           d.stream)) {
         // Result is in the alternate buffers: swap the roles so d.keys/d.vals hold it.
         std::swap(d.keys, d.keys_b);
         std::swap(d.vals, d.vals_b);

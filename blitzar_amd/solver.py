@@ -90,6 +90,12 @@ class OctreeHip:
             setattr(self.params, k, v)
         _check(lib().bh_hip_set_params(self._h, C.byref(self.params)), self._h, "set_params")
 
+    def resolved_params(self) -> Params:
+        """The configuration the handle runs with (after resolveForceLawPolicy)."""
+        out = Params()
+        _check(lib().bh_hip_get_params(self._h, C.byref(out)), self._h, "get_params")
+        return out
+
     def set_octree_theta(self, theta: float) -> None:
         self.set_params(theta=theta)
 
@@ -331,3 +337,31 @@ def uniform_cube(n: int, seed: int = 1, half_extent: float = 1.0, total_mass: fl
     lib().bh_ic_uniform_cube(n, seed, half_extent, total_mass, pos.reshape(-1), vel.reshape(-1),
                              m)
     return pos, vel, m
+
+
+def zeldovich_lattice(ng, seed=3, rms_cells=0.3, box=1.0):
+    """ng^3 lattice displaced by a first-order Zel'dovich field: Gaussian
+    random density with P(k) ~ k^-2, displacement psi_k = i k delta_k / k^2,
+    scaled to an RMS displacement of rms_cells lattice cells per axis."""
+    rng = np.random.default_rng(seed)
+    white = rng.standard_normal((ng, ng, ng)).astype(np.float32)
+    dk = np.fft.rfftn(white)
+    del white
+    k = np.fft.fftfreq(ng, d=1.0 / ng).astype(np.float32)
+    kz = np.arange(ng // 2 + 1, dtype=np.float32)
+    k2 = k[:, None, None] ** 2 + k[None, :, None] ** 2 + kz[None, None, :] ** 2
+    k2[0, 0, 0] = 1.0
+    dk *= 1.0 / np.sqrt(k2)  # amplitude sqrt(P) ~ 1/k
+    dk[0, 0, 0] = 0.0
+    g = (np.arange(ng, dtype=np.float64) + 0.5) / ng * box
+    pos = np.empty((ng ** 3, 3), np.float64)
+    disp = []
+    for a, K in enumerate((k[:, None, None], k[None, :, None], kz[None, None, :])):
+        comp = np.fft.irfftn(1j * K * dk / k2, s=(ng, ng, ng), axes=(0, 1, 2))
+        disp.append(comp.astype(np.float32).reshape(-1))
+        del comp
+    scale = rms_cells * (box / ng) / np.sqrt(np.mean([np.mean(d * d) for d in disp]))
+    grid = np.meshgrid(g, g, g, indexing="ij")
+    for a in range(3):
+        pos[:, a] = np.mod(grid[a].reshape(-1) + scale * disp[a], box)
+    return pos

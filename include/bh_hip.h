@@ -154,6 +154,17 @@ bh_hip_result_t bh_hip_create(const bh_hip_params_t* params, uint64_t n, bh_hip_
                               char* err, size_t err_cap);
 void bh_hip_destroy(bh_hip_handle_t* h);
 
+/* The configuration a handle runs with for `in`: the resolution above
+ * (resolveForceLawPolicy, engine/physics/core/force/PhyForceLawPolicy.cpp:22-32)
+ * plus the solver's own clamps (adaptive level <= 12 and eta in [0.01, 1] as
+ * CfgScenarioRuntime.cpp:63-68; far_alpha in [0, 1]; defaults for unset
+ * fields). Host only: needs no device. */
+void bh_hip_resolve_params(const bh_hip_params_t* in, bh_hip_params_t* out);
+
+/* The resolved configuration of a handle (the values ParticleSystem's
+ * getters return after configureSystem, engine/batch/runner/BatRunner.cpp:48-97). */
+bh_hip_result_t bh_hip_get_params(bh_hip_handle_t* h, bh_hip_params_t* out);
+
 /* Replace the configuration (same resolution rules as create). */
 bh_hip_result_t bh_hip_set_params(bh_hip_handle_t* h, const bh_hip_params_t* params);
 

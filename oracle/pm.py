@@ -36,17 +36,22 @@ def _cic(pos, L, N):
 
 
 def deposit(pos, mass, L, N):
-    rho = np.zeros((N, N, N))
+    rho = np.zeros(N * N * N)
     i0, f = _cic(pos, L, N)
+    mass = np.asarray(mass, np.float64)
     for dx in (0, 1):
         wx = f[:, 0] if dx else 1 - f[:, 0]
+        ix = (i0[:, 0] + dx) % N
         for dy in (0, 1):
             wy = f[:, 1] if dy else 1 - f[:, 1]
+            iy = (i0[:, 1] + dy) % N
             for dz in (0, 1):
                 wz = f[:, 2] if dz else 1 - f[:, 2]
-                np.add.at(rho, ((i0[:, 0] + dx) % N, (i0[:, 1] + dy) % N, (i0[:, 2] + dz) % N),
-                          mass * wx * wy * wz)
-    return rho / (L / N) ** 3
+                iz = (i0[:, 2] + dz) % N
+                # summed per cell in FP64 (bincount adds the weights of equal indices)
+                rho += np.bincount((ix * N + iy) * N + iz, weights=mass * wx * wy * wz,
+                                   minlength=N * N * N)
+    return rho.reshape(N, N, N) / (L / N) ** 3
 
 
 def interpolate(field, pos, L, N):
@@ -69,7 +74,9 @@ def wavenumbers(L, N):
     return k, kz
 
 
-def long_range_accel(pos, mass, L, N, rs, G=1.0):
+def long_range_accel(pos, mass, L, N, rs, G=1.0, targets=None):
+    """Long-range acceleration at every particle, or at pos[targets] only
+    (the deposit always takes every particle)."""
     rho_k = np.fft.rfftn(deposit(pos, mass, L, N))
     k, kz = wavenumbers(L, N)
     KX, KY, KZ = np.meshgrid(k, k, kz, indexing="ij")
@@ -80,10 +87,12 @@ def long_range_accel(pos, mass, L, N, rs, G=1.0):
     with np.errstate(divide="ignore", invalid="ignore"):
         phi_k = -4 * np.pi * G * rho_k * np.exp(-k2 * rs * rs) / k2 / (W * W)
     phi_k[0, 0, 0] = 0.0
-    acc = np.zeros((len(pos), 3))
+    del rho_k, W, k2
+    at = pos if targets is None else np.asarray(pos)[targets]
+    acc = np.zeros((len(at), 3))
     for a, K in enumerate((KX, KY, KZ)):
         field = np.fft.irfftn(-1j * K * phi_k, s=(N, N, N), axes=(0, 1, 2))
-        acc[:, a] = interpolate(field, pos, L, N)
+        acc[:, a] = interpolate(field, at, L, N)
     return acc
 
 

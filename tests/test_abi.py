@@ -105,3 +105,29 @@ def test_hernquist_profile():
     ke = 0.5 * (m * (v ** 2).sum(1)).sum()
     # K = |W| / 2 = 1 / 12 (G = M = a = 1)
     assert ke == pytest.approx(1 / 12, rel=0.06)
+
+
+def test_resolve_params_follows_the_force_law_policy(oracle):
+    """bh_hip_resolve_params (host only) against the oracle's restatement of
+    resolveForceLawPolicy (PhyForceLawPolicy.cpp:22-32), which is pinned
+    bit-exactly against the reference build."""
+    import blitzar_amd
+    from blitzar_amd import _abi
+    rng = np.random.default_rng(11)
+    cases = [(0.01, 0.0, 1e-4, 0.05), (9.0, 0.5, 1e-4, 0.05), (0.3, 1e-6, 1e-3, 0.4),
+             (0.7, 0.02, -1.0, 7.0), (0.5, 0.01, 1e-4, 0.0)]
+    cases += [tuple(float(v) for v in row) for row in
+              rng.uniform([0.0, 0.0, -0.01, -0.5], [5.0, 0.1, 0.05, 5.0], (40, 4))]
+    for theta, soft, min_soft, min_theta in cases:
+        p = blitzar_amd.default_params()
+        p.theta, p.softening, p.min_softening, p.min_theta = theta, soft, min_soft, min_theta
+        r = _abi.resolve_params(p)
+        th, so, _ = oracle.resolve_policy(oracle.make_params(
+            theta=theta, softening=soft, min_softening=min_soft, min_theta=min_theta))
+        assert r.theta == np.float32(th) and r.softening == np.float32(so)
+        assert r.min_softening == np.float32(max(0.0, min_soft))
+    # the solver's own clamps (CfgScenarioRuntime.cpp:63-68)
+    p = blitzar_amd.default_params()
+    p.adaptive_max_level, p.adaptive_eta, p.far_alpha = 20, 5.0, 3.0
+    r = _abi.resolve_params(p)
+    assert r.adaptive_max_level == 12 and r.adaptive_eta == 1.0 and r.far_alpha == 1.0

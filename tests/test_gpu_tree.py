@@ -205,10 +205,11 @@ def test_topology_with_a_dense_clump(oracle):
 def test_node_capacity_overflow_recovers(oracle):
     """Near-duplicate pairs split in single-child chains down to level 20:
     16,153 nodes for 2,000 particles, twice the initial capacity (2n + 4096).
-    The first build drops the level that would overflow (flagged, with the
-    level above standing in as leaves; every write stays inside the node
-    arrays), the capacity doubles, and the next build matches the oracle
-    exactly."""
+    The build that overflows drops the level that would not fit (flagged,
+    every write inside the node arrays); the call sees the flag right after
+    the build, doubles the capacity and builds again, so it returns the full
+    tree, equal to the oracle's. A force evaluation that meets the overflow
+    gives the forces of a handle that already had the capacity, bit for bit."""
     import blitzar_amd as B
     n = 2000
     rng = np.random.default_rng(9)
@@ -219,8 +220,6 @@ def test_node_capacity_overflow_recovers(oracle):
     say("case node capacity overflow n=2000")
     with _solver(n, leaf_capacity=16, bucket_capacity=1) as s:
         s.upload(pos, None, m)
-        with pytest.raises(B.BhHipError, match="node capacity"):
-            s.build_tree()
         s.build_tree()
         t = s.dump_tree()
     sk, _ = oracle.stable_sort(oracle.keys(pos, 0))
@@ -228,3 +227,45 @@ def test_node_capacity_overflow_recovers(oracle):
     assert len(t["level"]) == len(ref["level"]) == 16153
     for key in ("level", "prefix", "start", "count", "first_child", "nchild"):
         assert np.array_equal(t[key].astype(np.int64), ref[key].astype(np.int64)), key
+    say("case node capacity overflow inside a force evaluation and a step")
+    with _solver(n, leaf_capacity=16, bucket_capacity=1) as a, \
+            _solver(n, leaf_capacity=16, bucket_capacity=1) as b:
+        a.upload(pos, None, m)
+        b.upload(pos, None, m)
+        b.build_tree()  # grows b's capacity first
+        fa = a.compute_accelerations()
+        fb = b.compute_accelerations()
+        assert np.array_equal(fa, fb)
+        a.upload(pos, None, m)
+        c = _solver(n, leaf_capacity=16, bucket_capacity=1)
+        c.upload(pos, None, m)
+        c.step(1e-4, 3)   # overflows at its first build, recovers, completes
+        a.step(1e-4, 3)
+        pa, va = a.download()
+        pc, vc = c.download()
+        c.close()
+        assert np.array_equal(pa, pc) and np.array_equal(va, vc)
+
+
+def test_failed_node_growth_leaves_the_handle_unusable(monkeypatch):
+    """A node regrow whose allocation fails (forced by the test switch, which
+    asks for an impossible size) releases the node arrays and leaves the
+    handle unusable: the call and every later one return an error, none
+    touches the released memory."""
+    import blitzar_amd as B
+    n = 2000
+    rng = np.random.default_rng(9)
+    pos = np.repeat(rng.random((n // 2, 3)), 2, axis=0)
+    pos[1::2] += 1.0e-6
+    m = np.full(n, 1.0 / n, np.float32)
+    say("case failed node growth n=2000")
+    monkeypatch.setenv("BH_HIP_TEST_FAIL_NODE_GROWTH", "1")
+    with _solver(n, leaf_capacity=16, bucket_capacity=1) as s:
+        monkeypatch.delenv("BH_HIP_TEST_FAIL_NODE_GROWTH")
+        s.upload(pos, None, m)
+        with pytest.raises(B.BhHipError, match="unusable"):
+            s.compute_accelerations()
+        for call in (lambda: s.step(1e-3, 1), s.download, s.build_tree, s.dump_tree,
+                     lambda: s.upload(pos, None, m)):
+            with pytest.raises(B.BhHipError, match="unusable"):
+                call()

@@ -71,3 +71,47 @@ def test_directive_grammar(tmp_path):
     assert "criterion=bounds" in eff and "bucket=8, order=1" in eff
     assert 'file="' in eff and "a, b.bin" in eff
     assert "malformed directive" in r.stderr
+
+
+def resolved(out):
+    line = [ln for ln in out.splitlines() if ln.startswith("[headless] resolved ")][0]
+    return dict(kv.split("=") for kv in line.split()[2:])
+
+
+def test_theta_is_clamped_into_the_auto_bounds(tmp_path):
+    """resolveOctreeTheta (SrvFormatAndTheta.cpp:54-69) clamps theta into
+    [theta_auto_min, theta_auto_max] even with auto-tune off, then
+    resolveForceLawPolicy clamps it into [min_theta, 4]."""
+    ini = tmp_path / "c.ini"
+    cases = [("theta=0.3, theta_auto_min=0.4, theta_auto_max=1.2", 0.4),
+             ("theta=1.5, theta_auto_min=0.4, theta_auto_max=1.2", 1.2),
+             ("theta=0.7, theta_auto_min=0.4, theta_auto_max=1.2", 0.7),
+             ("theta=0.3, theta_auto_min=0.01, theta_auto_max=1.2", 0.3),
+             ("theta=0.01, theta_auto_min=0.01, theta_auto_max=0.02", 0.05),
+             ("theta=0.3, theta_auto_min=0.9, theta_auto_max=0.5", 0.9)]
+    for octree, want in cases:
+        ini.write_text(f"simulation(solver=octree_hip)\noctree({octree}, softening=0.02)\n"
+                       "physics(min_softening=0.05)\n")
+        r = run("--config", str(ini), "--inspect")
+        assert r.returncode == 0, r.stderr
+        got = resolved(r.stdout)
+        assert float(got["theta"]) == pytest.approx(want, rel=1e-6), octree
+        assert float(got["softening"]) == pytest.approx(0.05, rel=1e-6)  # max(eps, min_softening)
+
+
+@pytest.mark.parametrize("payload", ["truncated", "oversized", "too_few"])
+def test_snapshot_header_counts_are_checked_before_allocating(tmp_path, payload):
+    """A corrupt NBSIMBIN header must fail the parse (and fall back to the
+    generated scene, as the reference does) instead of asking for gigabytes."""
+    import struct
+    path = tmp_path / "bad.bin"
+    count = {"truncated": 1000, "oversized": 0xFFFFFFF0, "too_few": 1}[payload]
+    body = b"\0" * 32 * (10 if payload != "too_few" else 1)
+    path.write_bytes(b"NBSIMBIN" + struct.pack("<II", 1, count) + body)
+    ini = tmp_path / "c.ini"
+    ini.write_text("simulation(particle_count=64, dt=0.001, solver=octree_hip)\n"
+                   f"scene(style=file, file={path}, format=bin)\n")
+    r = run("--config", str(ini), "--run", "--target-steps", "1")
+    assert "failed to parse input state" in r.stderr
+    assert "outside [2, 2^31 - 1] or beyond the file size" in r.stderr
+    assert r.returncode in (0, 5)  # 5 where no GPU is present; never an abort
