@@ -124,6 +124,37 @@ def walk_traffic(args, n):
     return None
 
 
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def hbm_fractions(tm, n, walks, args):
+    """Algorithmic bytes over device time, against 8 TB/s, for the
+    bandwidth-bound phases of one step (HIP events on the library's stream
+    around each phase, averaged over the timed force evaluations):
+      keys       36 B/particle (FP64 positions in, 63-bit key + index out);
+      sort      192 B/particle (8 passes x 24 B, the fixed 8-bit-digit
+                convention, whatever the pass count);
+      permute   116 B/particle (+8 B when the relative criterion carries |a|);
+      tree       40 B/particle + 128 B/node (levels, moments, leaf packing
+                and walk groups together);
+      integrate 108 B/particle per kick + drift (the closing half kick of a
+                call, 60 B, is in the time but not in the bytes)."""
+    nodes = float(tm["node_count"])
+    ph = {
+        "keys": (36.0 * n, tm["keys_ms"] / walks),
+        "sort": (192.0 * n, tm["sort_ms"] / walks),
+        "permute": ((124.0 if args.criterion == "relative" else 116.0) * n,
+                    tm["permute_ms"] / walks),
+        "tree": (40.0 * n + 128.0 * nodes, (tm["tree_ms"] + tm["moments_ms"]) / walks),
+        "integrate": (108.0 * n, tm["integrate_ms"] / max(1, args.steps)),
+    }
+    out = {}
+    for k, (b, ms) in ph.items():
+        rate = b / (ms * 1e-3) if ms > 0 else 0.0
+        out[k] = {"bytes": b, "ms": ms, "tb_per_s": rate / 1e12, "frac": rate / HBM_PEAK}
+    return out
+
+
 def workload_name(args, n):
     crit = (f"relative{args.relative_alpha}_mono{args.relative_mono}" if args.criterion == "relative"
             else args.criterion)
@@ -330,11 +361,12 @@ def main(argv=None):
                     help="group far-field expansion (0 = off)")
     ap.add_argument("--ref-theta", type=float, default=0.5)
     ap.add_argument("--ref-steps", type=int, default=5)
-    ap.add_argument("--lff", action="store_true")
-    ap.add_argument("--block-steps", type=int, default=0,
-                    help="also run N base steps with dyadic block time steps")
-    ap.add_argument("--treepm", type=int, default=0,
-                    help="also run N steps of the periodic TreePM box (256^3 Zel'dovich lattice)")
+    ap.add_argument("--no-lff", action="store_true",
+                    help="skip the learned far-field correction (theta 0.9 + MLP)")
+    ap.add_argument("--block-steps", type=int, default=64,
+                    help="base steps of the dyadic block time-step workload (0 = skip)")
+    ap.add_argument("--treepm", type=int, default=10,
+                    help="steps of the periodic TreePM box, 256^3 Zel'dovich lattice (0 = skip)")
     ap.add_argument("--eps", type=float, default=0.01)
     ap.add_argument("--order", type=int, default=2)
     ap.add_argument("--bucket", type=int, default=16)
@@ -417,7 +449,7 @@ def main(argv=None):
         log(f"[bench] block time steps: {block_line}")
 
     lff_line = None
-    if args.lff:
+    if not args.no_lff:
         lff_line = learned_far_field(s, args, n, m, t, ref is not None)
         log(f"[bench] learned far field: {lff_line}")
 
@@ -469,8 +501,9 @@ def main(argv=None):
             "build": tm["build_ms"] / walks, "sort": tm["sort_ms"] / walks,
             "walk": walk_ms, "integrate": tm["integrate_ms"] / max(1, args.steps),
         },
+        "hbm_fraction": hbm_fractions(tm, n, walks, args),
         "roofline": {
-            "kernel": "walk_kernel", "bound": "mfma", "pipe": "fp32 VALU",
+            "kernel": "walk_kernel", "bound": "valu", "pipe": "fp32 VALU",
             "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS, "traffic": walk_traffic(args, n),
             "traffic_source": f"profiles/{os.path.basename(PMC_SUMMARY)} (rocprofv3 PMC)",

@@ -64,6 +64,10 @@ class Timings(C.Structure):
         ("far_interactions", C.c_uint64),
         ("far_evaluations", C.c_uint64),
         ("mono_interactions", C.c_uint64),
+        ("keys_ms", C.c_double),
+        ("permute_ms", C.c_double),
+        ("tree_ms", C.c_double),
+        ("moments_ms", C.c_double),
     ]
 
 

@@ -25,8 +25,12 @@ constexpr int kDefaultWavesPerCu = 24;
 constexpr float kDefaultRelativeAlpha = 0.002f;
 constexpr size_t kStackCap = 16384;
 
+// b0 build start, s0 sort start (after AABB + keys), s1 sort end, p1 after
+// the tie fix and permutation, t1 after the tree levels, b1 build end (after
+// moments and groups), w1 walk end.
 struct StepEvents {
-    hipEvent_t b0 = nullptr, s0 = nullptr, s1 = nullptr, b1 = nullptr, w1 = nullptr;
+    hipEvent_t b0 = nullptr, s0 = nullptr, s1 = nullptr, p1 = nullptr, t1 = nullptr,
+               b1 = nullptr, w1 = nullptr;
 };
 
 // Start and end of one kick / kick-drift launch (read at the end of the call,
@@ -103,8 +107,13 @@ bool dalloc(bh_hip_handle* h, T** p, size_t count, const char* what)
 {
     void* q = nullptr;
     const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-    if (!check(h, hipMalloc(&q, bytes), what))
+    if (!check(h, hipMalloc(&q, bytes), what)) {
+        // HIP keeps the failure as the thread's last error until it is read;
+        // read it here so that the next launch check of this or another
+        // handle does not report it again
+        (void)hipGetLastError();
         return false;
+    }
     h->allocations.push_back(q);
     *p = static_cast<T*>(q);
     return true;
@@ -150,7 +159,7 @@ bh_hip_params_t resolve(const bh_hip_params_t& in)
     // CfgScenarioRuntime.cpp:63-68: max level <= 12, eta in [0.01, 1]
     p.adaptive_max_level = std::clamp(in.adaptive_max_level, 0, 12);
     p.adaptive_eta = std::clamp(in.adaptive_eta, 0.01f, 1.0f);
-    p.adaptive_flags = in.adaptive_flags & BH_HIP_ADAPTIVE_VELOCITY;
+    p.adaptive_flags = in.adaptive_flags & (BH_HIP_ADAPTIVE_VELOCITY | BH_HIP_ADAPTIVE_REBUILD);
     p.periodic_box = std::max(0.0f, in.periodic_box);
     p.pm_mesh = std::max(0, in.pm_mesh);
     p.far_alpha = std::clamp(in.far_alpha, 0.0f, 1.0f);
@@ -201,6 +210,7 @@ bool alloc_nodes(bh_hip_handle* h, size_t cap)
     dfree(h, d.nfirst);
     dfree(h, d.nlevel);
     dfree(h, d.wnode);
+    dfree(h, d.next);
     dfree(h, d.scan_tmp);
     dfree(h, d.bnd_tmp);
     dfree(h, d.big_list);
@@ -213,12 +223,14 @@ bool alloc_nodes(bh_hip_handle* h, size_t cap)
     d.scan_blocks = d.nparent = d.gscan = d.gscan_blocks = nullptr;
     d.nlevel = nullptr;
     d.wnode = nullptr;
+    d.next = nullptr;
     d.node_cap = cap;
     const size_t scan_n = std::min(d.n, cap);
     return dalloc(h, &d.nprefix, cap, "node prefix") && dalloc(h, &d.nstart, cap, "node start") &&
            dalloc(h, &d.ncount, cap, "node count") && dalloc(h, &d.nchild, cap, "node nchild") &&
            dalloc(h, &d.nfirst, cap, "node first") && dalloc(h, &d.nlevel, cap, "node level") &&
-           dalloc(h, &d.wnode, cap, "walk nodes") && dalloc(h, &d.scan_tmp, scan_n, "scan tmp") &&
+           dalloc(h, &d.wnode, cap, "walk nodes") && dalloc(h, &d.next, cap, "node extent") &&
+           dalloc(h, &d.scan_tmp, scan_n, "scan tmp") &&
            dalloc(h, &d.bnd_tmp, 8 * scan_n, "child bounds") &&
            dalloc(h, &d.big_list, scan_n, "large-node list") &&
            dalloc_zero(h, &d.scan_blocks, bh::scan_block_sums_words(scan_n), "scan blocks",
@@ -372,10 +384,9 @@ StepEvents* next_events(bh_hip_handle* h)
         return nullptr;
     if (h->events_used == h->events.size()) {
         StepEvents e;
-        if (hipEventCreate(&e.b0) != hipSuccess || hipEventCreate(&e.s0) != hipSuccess ||
-            hipEventCreate(&e.s1) != hipSuccess || hipEventCreate(&e.b1) != hipSuccess ||
-            hipEventCreate(&e.w1) != hipSuccess)
-            return nullptr;
+        for (hipEvent_t* x : {&e.b0, &e.s0, &e.s1, &e.p1, &e.t1, &e.b1, &e.w1})
+            if (hipEventCreate(x) != hipSuccess)
+                return nullptr;
         h->events.push_back(e);
     }
     return &h->events[h->events_used++];
@@ -416,8 +427,10 @@ void queue_build(bh_hip_handle* h, StepEvents* ev)
     rec(ev ? ev->s1 : nullptr, d.stream);
     bh::launch_tie_fix(d);
     bh::launch_permute(d);
+    rec(ev ? ev->p1 : nullptr, d.stream);
     h->rank_valid = false;
     bh::launch_tree_build(d, h->leaf_depth, h->p.bucket_capacity);
+    rec(ev ? ev->t1 : nullptr, d.stream);
     if (h->p.key_mode == 0) {
         bh::launch_tree_moments(d);
         bh::launch_groups(d);
@@ -426,6 +439,20 @@ void queue_build(bh_hip_handle* h, StepEvents* ev)
     (void)hipMemcpyAsync(h->err_host, d.err, sizeof(int), hipMemcpyDeviceToHost, d.stream);
     (void)hipEventRecord(h->built, d.stream);
     h->have_tree = true;
+}
+
+// Moments of the current positions on the topology of the last build (block
+// time-step slices between rebuilds): no sort, no permutation, no new nodes.
+void queue_refit(bh_hip_handle* h, StepEvents* ev)
+{
+    bh::Device& d = h->d;
+    for (hipEvent_t e : {ev ? ev->b0 : nullptr, ev ? ev->s0 : nullptr, ev ? ev->s1 : nullptr,
+                         ev ? ev->p1 : nullptr, ev ? ev->t1 : nullptr})
+        rec(e, d.stream);
+    bh::launch_tree_moments(d, true);
+    rec(ev ? ev->b1 : nullptr, d.stream);
+    (void)hipMemcpyAsync(h->err_host, d.err, sizeof(int), hipMemcpyDeviceToHost, d.stream);
+    (void)hipEventRecord(h->built, d.stream);
 }
 
 // A build with nothing queued behind it (topology dumps), regrown and
@@ -576,13 +603,18 @@ void end_call(bh_hip_handle* h)
     if (h->timing) {
         for (size_t k = 0; k < h->events_used; ++k) {
             const StepEvents& e = h->events[k];
-            float a = 0.0f, b = 0.0f, c = 0.0f;
-            if (hipEventElapsedTime(&a, e.b0, e.b1) == hipSuccess)
-                t.build_ms += a;
-            if (hipEventElapsedTime(&b, e.s0, e.s1) == hipSuccess)
-                t.sort_ms += b;
-            if (hipEventElapsedTime(&c, e.b1, e.w1) == hipSuccess)
-                t.walk_ms += c;
+            auto span = [](hipEvent_t x, hipEvent_t y, double& acc) {
+                float v = 0.0f;
+                if (hipEventElapsedTime(&v, x, y) == hipSuccess)
+                    acc += v;
+            };
+            span(e.b0, e.b1, t.build_ms);
+            span(e.b0, e.s0, t.keys_ms);
+            span(e.s0, e.s1, t.sort_ms);
+            span(e.s1, e.p1, t.permute_ms);
+            span(e.p1, e.t1, t.tree_ms);
+            span(e.t1, e.b1, t.moments_ms);
+            span(e.b1, e.w1, t.walk_ms);
         }
         for (size_t k = 0; k < h->int_used; ++k) {
             float a = 0.0f;
@@ -655,12 +687,16 @@ bool alloc_adaptive(bh_hip_handle* h)
     return true;
 }
 
-// Build + walk of one block-step force evaluation, regrown and repeated on a
-// node-capacity overflow.
-bool build_and_walk(bh_hip_handle* h, StepEvents* ev, const bh::WalkParams& wp)
+// Build (or refit) + walk of one block-step force evaluation, regrown and
+// repeated on a node-capacity overflow (a refit makes no nodes).
+bool build_and_walk(bh_hip_handle* h, StepEvents* ev, const bh::WalkParams& wp,
+                    bool refit = false)
 {
     for (int attempt = 0; attempt < 8; ++attempt) {
-        queue_build(h, ev);
+        if (refit)
+            queue_refit(h, ev);
+        else
+            queue_build(h, ev);
         launch_force_walk(h, wp);
         const Built b = check_build(h);
         if (b == Built::kOk) {
@@ -709,10 +745,17 @@ bh_hip_result_t step_adaptive(bh_hip_handle* h, double dt, int32_t nsteps)
     // single-slice predictors one predictor over k q).
     uint32_t* due = reinterpret_cast<uint32_t*>(d.diag);
     const uint64_t slices = 1ull << a.levels;
+    // The tree is built at the first force evaluation of each global step
+    // and refitted (moments of the predicted positions on the same topology)
+    // at the later ones, as the reference keeps one tree per global step
+    // (CudOctreeGpuAdaptive.inl:89-113,147); BH_HIP_ADAPTIVE_REBUILD builds
+    // at every one.
+    const bool rebuild_each = (h->p.adaptive_flags & BH_HIP_ADAPTIVE_REBUILD) != 0;
     h->steps_done = 0;
     bool stop = false;
     for (int k = 0; k < nsteps && !stop; ++k) {
         const uint64_t end = h->ad_tick + slices;
+        bool first_slice = true;
         while (h->ad_tick < end) {
             const uint32_t t0 = static_cast<uint32_t>(h->ad_tick);
             bh::launch_adaptive_next_due(d, a, t0, due);
@@ -730,8 +773,9 @@ bh_hip_result_t step_adaptive(bh_hip_handle* h, double dt, int32_t nsteps)
             wp.alevel = d.alevel;
             wp.atick = tt;
             wp.alevels = a.levels;
-            if (!build_and_walk(h, next_events(h), wp))
+            if (!build_and_walk(h, next_events(h), wp, !first_slice && !rebuild_each))
                 return fail();
+            first_slice = false;
             bh::launch_adaptive_correct(d, a, tt, h->stats + 2);
             if (bits_at_build(h) & kFatalBits) {
                 stop = true;
@@ -876,7 +920,7 @@ void bh_hip_destroy(bh_hip_handle_t* h)
         (void)hipFree(p);
     h->allocations.clear();
     for (auto& e : h->events) {
-        for (hipEvent_t x : {e.b0, e.s0, e.s1, e.b1, e.w1})
+        for (hipEvent_t x : {e.b0, e.s0, e.s1, e.p1, e.t1, e.b1, e.w1})
             if (x)
                 (void)hipEventDestroy(x);
     }

@@ -191,6 +191,7 @@ struct Device {
     uint8_t* nlevel = nullptr;
     uint32_t* nparent = nullptr;  // parent index (root: 0xffffffff)
     WalkNode* wnode = nullptr;
+    float* next = nullptr;            // per node: particle extent about the lattice centre
     uint32_t* gscan = nullptr;        // per-node group counts -> offsets
     uint32_t* gscan_blocks = nullptr;
     uint4* groups = nullptr;          // walk groups: first particle, count, node
@@ -288,7 +289,11 @@ void launch_permute(Device& d);
 void launch_rank(Device& d);  // rebuild rank from id
 // k_tree.hip
 void launch_tree_build(Device& d, int leaf_depth, int bucket);
-void launch_tree_moments(Device& d);
+// Moments, leaf packing and the walk's node records of the current
+// positions. refit: the topology is that of an earlier build (positions have
+// moved since); each node's level in the walk record is lowered until its
+// cell covers the node's particles.
+void launch_tree_moments(Device& d, bool refit = false);
 // Walk groups: a partition of the sorted particles into spatially compact
 // runs of at most kGroupMax (consecutive small children of one node packed
 // together; see node_groups in k_tree.hip).

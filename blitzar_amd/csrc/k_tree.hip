@@ -350,7 +350,8 @@ __global__ __launch_bounds__(256) void tree_moments_kernel(
     const uint32_t* __restrict__ nfirst, const int* __restrict__ info, int level,
     const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
     const float* __restrict__ pm, const RootGeom* __restrict__ geom, WalkNode* __restrict__ wn,
-    float4* __restrict__ spart, uint32_t* __restrict__ sleaf, int* err)
+    float4* __restrict__ spart, uint32_t* __restrict__ sleaf, float* __restrict__ ext,
+    int refit, int* err)
 {
     const int base = info[level];
     const int cnt = info[kLevels + level];
@@ -363,6 +364,10 @@ __global__ __launch_bounds__(256) void tree_moments_kernel(
         const uint32_t first = nfirst[node];
         float M = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
         float qxx = 0.0f, qxy = 0.0f, qxz = 0.0f, qyy = 0.0f, qyz = 0.0f, bmax = 0.0f, trs = 0.0f;
+        // e: half-width, in lattice units along the widest axis, of the cube
+        // about the lattice centre that holds the subtree's particles (the
+        // cell's own half-width 2^(21 - level) unless they left the cell)
+        float e = 0.0f;
         uint32_t count = 0;
         if (nc == 0) {
             count = ncount[node];
@@ -378,6 +383,7 @@ __global__ __launch_bounds__(256) void tree_moments_kernel(
                 const float m = pm[s];
                 spart[s] = make_float4(x, y, z, m);
                 sleaf[s] = static_cast<uint32_t>(node);
+                e = fmaxf(e, fmaxf(fabsf(x) / ux, fmaxf(fabsf(y) / uy, fabsf(z) / uz)));
                 M += m;
                 sx += m * x;
                 sy += m * y;
@@ -440,7 +446,21 @@ __global__ __launch_bounds__(256) void tree_moments_kernel(
                 trs += live ? ch.trs + ch.mass * r2 : 0.0f;
                 if (live && ch.mass > 0.0f)
                     bmax = fmaxf(bmax, sqrtf(r2) + ch.bmax);
+                if (live) {
+                    const int off = max(abs(ch.cx - c[0]), max(abs(ch.cy - c[1]), abs(ch.cz - c[2])));
+                    e = fmaxf(e, ext[first + jj] + static_cast<float>(off));
+                }
             }
+        }
+        ext[node] = e;
+        // Refit (moments of the current positions on an earlier topology):
+        // particles may have left their cells, so the level the walk derives
+        // the cell size from is lowered until that cell's half-width covers
+        // e. A fresh build keeps the true level.
+        int wlevel = level;
+        if (refit) {
+            while (wlevel > 0 && ldexpf(1.0f, kMaxLevel - wlevel) < e)
+                --wlevel;
         }
         WalkNode w;
         w.comx = cx;
@@ -468,7 +488,7 @@ __global__ __launch_bounds__(256) void tree_moments_kernel(
             atomicOr(err, kErrLeafTooLarge);
             count = kMaxLeafCount;
         }
-        w.meta = make_meta(nc, static_cast<uint32_t>(level), count);
+        w.meta = make_meta(nc, static_cast<uint32_t>(wlevel), count);
         wn[node] = w;
     }
 }
@@ -638,13 +658,14 @@ void launch_groups(Device& d)
                                                d.nparent, d.gscan, d.level_info, d.groups);
 }
 
-void launch_tree_moments(Device& d)
+void launch_tree_moments(Device& d, bool refit)
 {
     for (int L = kMaxLevel; L >= 0; --L) {
         const size_t bound = level_bound(L, d.n, d.node_cap);
         tree_moments_kernel<<<level_grid(bound), 256, 0, d.stream>>>(
             d.nprefix, d.nstart, d.ncount, d.nchild, d.nfirst, d.level_info, L, d.pos[0],
-            d.pos[1], d.pos[2], d.mass, d.geom, d.wnode, d.spart, d.sleaf, d.err);
+            d.pos[1], d.pos[2], d.mass, d.geom, d.wnode, d.spart, d.sleaf, d.next,
+            refit ? 1 : 0, d.err);
     }
 }
 

@@ -117,8 +117,15 @@ typedef struct bh_hip_params {
     float relative_mono;
 } bh_hip_params_t;
 
-/* adaptive_flags: also bound the step by eta * eps / |v| (the reference does) */
-enum { BH_HIP_ADAPTIVE_VELOCITY = 1 };
+/* adaptive_flags:
+ *   VELOCITY: also bound the step by eta * eps / |v| (the reference does);
+ *   REBUILD:  build the tree at every force evaluation of a global step. By
+ *             default it is built at the first one and refitted at the others
+ *             (moments of the predicted positions on the same topology, cell
+ *             sizes widened to cover particles that left their cells), as the
+ *             reference keeps one tree per global step
+ *             (CudOctreeGpuAdaptive.inl:89-113,147). */
+enum { BH_HIP_ADAPTIVE_VELOCITY = 1, BH_HIP_ADAPTIVE_REBUILD = 2 };
 
 typedef struct bh_hip_handle bh_hip_handle_t;
 
@@ -142,6 +149,11 @@ typedef struct bh_hip_timings {
     uint64_t far_evaluations;   /* group far field: per-target expansion evaluations */
     uint64_t mono_interactions; /* relative_mono: accepted nodes evaluated as point masses
                                    (per target; also counted in pp_interactions) */
+    /* parts of build_ms */
+    double keys_ms;     /* AABB + Morton keys */
+    double permute_ms;  /* equal-key tie order + store permutation */
+    double tree_ms;     /* tree levels (topology) */
+    double moments_ms;  /* moments, leaf packing and walk groups */
 } bh_hip_timings_t;
 
 /* Fill `out` with the defaults (reference config defaults, CfgConfig.hpp). */
