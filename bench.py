@@ -155,6 +155,21 @@ def hbm_fractions(tm, n, walks, args):
     return out
 
 
+def copy_calibration(mib):
+    """A plain device-to-device copy of `mib` MiB, three times: each dispatch
+    reads and writes exactly mib MiB, which fixes the scale of the profiler's
+    byte counters for the other kernels of the same run."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    log(f"[bench] copy calibration: 3 copies of {mib} MiB")
+
+
 def workload_name(args, n):
     crit = (f"relative{args.relative_alpha}_mono{args.relative_mono}" if args.criterion == "relative"
             else args.criterion)
@@ -308,17 +323,22 @@ def block_time_steps(s, args, n):
     return out
 
 
-def learned_far_field(s, args, n, m_now, t, with_error):
-    """theta 0.9 walk with the learned correction: trained on a second
-    realisation (seed 12) of the workload, applied to the timed state."""
+def learned_far_field(s, args, n, m_now, t, with_error, t05_ms):
+    """theta 0.9 walk with the learned correction (32 -> 64 -> 64 -> 3 SiLU
+    MLP over the node's moments up to third order and the target geometry,
+    BF16 MFMA), trained on a second realisation (seed 12) of the workload,
+    applied to the timed state. Reported beside the same walk with a zero
+    network (its near nodes of < min_count particles opened, the rest
+    uncorrected) and with the ratio to the theta 0.5 walk."""
     import blitzar_amd as B
+    band, min_count = args.lff_band, 64
     p_now, v_now = s.download()
     if args.workload == "merger":
         p_tr, v_tr, m_tr = B.hernquist_merger(n // 2, seed=12, separation=20.0, pericentre=2.0)
     else:
         p_tr, v_tr, m_tr = B.plummer(n, seed=12)
     s.set_params(theta=0.9)
-    s.lff_configure(theta=0.9, band=0.5, min_count=64)
+    s.lff_configure(theta=0.9, band=band, min_count=min_count)
     s.upload(p_tr, v_tr, m_tr)
     t0 = time.perf_counter()
     tuples = s.lff_collect(group_stride=1, max_tuples=1 << 24)
@@ -326,20 +346,35 @@ def learned_far_field(s, args, n, m_now, t, with_error):
     t0 = time.perf_counter()
     loss = s.lff_train(epochs=6, batch=16384, lr=2e-3, seed=3)
     t_train = time.perf_counter() - t0
-    s.upload(p_now, v_now, m_now)
-    s.lff_enable(True)
-    s.step(args.dt, 1)
-    t0 = time.perf_counter()
-    s.step(args.dt, args.ref_steps)
-    el = time.perf_counter() - t0
-    tm = s.timings()
-    out = {"theta": 0.9, "band": 0.5, "min_count": 64, "tuples": tuples,
-           "collect_s": t_collect, "train_s": t_train, "final_loss": float(loss[-1]),
-           "value": n * args.ref_steps / el, "ms_per_step": el * 1e3 / args.ref_steps,
-           "walk_and_mlp_ms": tm["walk_ms"] / max(1, tm["walks"])}
-    if with_error:
-        out["force_error"] = force_error(s.compute_accelerations()[t], s.direct_forces_fp64(t),
-                                         len(t))
+    params = s.lff_params()
+    out = {"theta": 0.9, "band": band, "min_count": min_count, "tuples": tuples,
+           "collect_s": t_collect, "train_s": t_train, "final_loss": float(loss[-1])}
+    for tag, p in (("zero_network", np.zeros_like(params)), ("learned", params)):
+        s.lff_set_params(p)
+        s.lff_enable(True)
+        s.upload(p_now, v_now, m_now)
+        err = None
+        if with_error:
+            err = force_error(s.compute_accelerations()[t], s.direct_forces_fp64(t), len(t))
+        s.step(args.dt, 1)
+        t0 = time.perf_counter()
+        s.step(args.dt, args.ref_steps)
+        el = time.perf_counter() - t0
+        tm = s.timings()
+        w = max(1, tm["walks"])
+        line = {"value": n * args.ref_steps / el, "ms_per_step": el * 1e3 / args.ref_steps,
+                "walk_and_mlp_ms": tm["walk_ms"] / w, "mlp_ms": tm["lff_ms"] / w,
+                "pairs_per_walk": tm["lff_pairs"] / w, "tiles_per_walk": tm["lff_tiles"] / w,
+                "ratio_to_theta_0.5": t05_ms / (el * 1e3 / args.ref_steps),
+                "force_error": err}
+        if tag == "learned":
+            # MLP FLOP: 2 (32 x 64 + 64 x 64 + 64 x 16) per pair slot of a
+            # 16-pair tile, on the matrix cores
+            flop = 2.0 * (32 * 64 + 64 * 64 + 64 * 16) * 16 * tm["lff_tiles"] / w
+            line["mfma_tflops"] = flop / max(tm["lff_ms"] / w * 1e-3, 1e-12) / 1e12
+            out.update(line)
+        else:
+            out[tag] = line
     s.lff_enable(False)
     return out
 
@@ -363,6 +398,8 @@ def main(argv=None):
     ap.add_argument("--ref-steps", type=int, default=5)
     ap.add_argument("--no-lff", action="store_true",
                     help="skip the learned far-field correction (theta 0.9 + MLP)")
+    ap.add_argument("--lff-band", type=float, default=0.75,
+                    help="learned correction: near nodes with size / distance >= band")
     ap.add_argument("--block-steps", type=int, default=64,
                     help="base steps of the dyadic block time-step workload (0 = skip)")
     ap.add_argument("--treepm", type=int, default=10,
@@ -377,6 +414,10 @@ def main(argv=None):
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the 65,536 and 1,048,576 Plummer configurations")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run a plain device-to-device copy of MIB MiB three times "
+                         "(reads and writes MIB MiB each): the known-size kernel that "
+                         "calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -386,6 +427,9 @@ def main(argv=None):
         return 2
 
     import blitzar_amd as B
+
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
 
     t0 = time.perf_counter()
     pos, vel, m = make_state(args.workload, args.n)
@@ -450,7 +494,8 @@ def main(argv=None):
 
     lff_line = None
     if not args.no_lff:
-        lff_line = learned_far_field(s, args, n, m, t, ref is not None)
+        lff_line = learned_far_field(s, args, n, m, t, ref is not None,
+                                     ref_line["ms_per_step"])
         log(f"[bench] learned far field: {lff_line}")
 
     cpu = None

@@ -68,6 +68,9 @@ class Timings(C.Structure):
         ("permute_ms", C.c_double),
         ("tree_ms", C.c_double),
         ("moments_ms", C.c_double),
+        ("lff_pairs", C.c_uint64),
+        ("lff_tiles", C.c_uint64),
+        ("lff_ms", C.c_double),
     ]
 
 
@@ -81,10 +84,10 @@ class LffConfig(C.Structure):
     ]
 
 
-LFF_INPUTS = 16
+LFF_INPUTS = 32
 LFF_HIDDEN = 64
 LFF_OUTPUTS = 3
-LFF_PARAMS = 5443
+LFF_PARAMS = 6467
 
 
 # Every exported symbol of include/bh_hip.h and include/bh_ic.h.

@@ -31,6 +31,8 @@ constexpr size_t kStackCap = 16384;
 struct StepEvents {
     hipEvent_t b0 = nullptr, s0 = nullptr, s1 = nullptr, p1 = nullptr, t1 = nullptr,
                b1 = nullptr, w1 = nullptr;
+    hipEvent_t l0 = nullptr;  // learned correction start (after its walk)
+    bool l0_used = false;
 };
 
 // Start and end of one kick / kick-drift launch (read at the end of the call,
@@ -70,7 +72,9 @@ struct bh_hip_handle {
     // node regrow asks for an impossible size, so the failed-allocation path
     // runs for real.
     bool test_fail_growth = false;
-    unsigned long long* stats = nullptr;  // device: pp, pc, block-step particle steps
+    // device counters: pp, pc, block-step particle steps, far node-group,
+    // far evaluations, point-mass cells, learned-correction pairs and tiles
+    unsigned long long* stats = nullptr;
     std::vector<StepEvents> events;
     size_t events_used = 0;
     std::vector<IntegrationEvents> int_events;
@@ -218,6 +222,9 @@ bool alloc_nodes(bh_hip_handle* h, size_t cap)
     dfree(h, d.nparent);
     dfree(h, d.gscan);
     dfree(h, d.gscan_blocks);
+    const bool oct = d.noct != nullptr;
+    dfree(h, d.noct);
+    d.noct = nullptr;
     d.nprefix = nullptr;
     d.nstart = d.ncount = d.nchild = d.nfirst = d.scan_tmp = d.bnd_tmp = d.big_list = nullptr;
     d.scan_blocks = d.nparent = d.gscan = d.gscan_blocks = nullptr;
@@ -237,7 +244,8 @@ bool alloc_nodes(bh_hip_handle* h, size_t cap)
                        d.stream) &&
            dalloc(h, &d.nparent, cap, "node parent") && dalloc(h, &d.gscan, cap, "group scan") &&
            dalloc_zero(h, &d.gscan_blocks, bh::scan_block_sums_words(cap), "group scan blocks",
-                       d.stream);
+                       d.stream) &&
+           (!oct || dalloc(h, &d.noct, 10 * cap, "node third moments"));
 }
 
 bool alloc_all(bh_hip_handle* h, uint64_t n)
@@ -266,7 +274,7 @@ bool alloc_all(bh_hip_handle* h, uint64_t n)
     if (!dalloc(h, &d.level_info, 2 * bh::kLevels + 8, "level info") ||
         !dalloc(h, &d.geom, 1, "geometry") || !dalloc(h, &d.aabb_bits, 8, "aabb") ||
         !dalloc(h, &d.err, 4, "error flags") || !dalloc(h, &d.work_counter, bh::kTicketWords, "work counter") ||
-        !dalloc(h, &h->stats, 6, "walk stats"))
+        !dalloc(h, &h->stats, 8, "walk stats"))
         return false;
     d.diag_cap = 6 * 1024 + 8 * 256 + 16;  // AABB partials, energy partials
     if (!dalloc(h, &d.diag, d.diag_cap, "diagnostics"))
@@ -384,12 +392,14 @@ StepEvents* next_events(bh_hip_handle* h)
         return nullptr;
     if (h->events_used == h->events.size()) {
         StepEvents e;
-        for (hipEvent_t* x : {&e.b0, &e.s0, &e.s1, &e.p1, &e.t1, &e.b1, &e.w1})
+        for (hipEvent_t* x : {&e.b0, &e.s0, &e.s1, &e.p1, &e.t1, &e.b1, &e.w1, &e.l0})
             if (hipEventCreate(x) != hipSuccess)
                 return nullptr;
         h->events.push_back(e);
     }
-    return &h->events[h->events_used++];
+    StepEvents* ev = &h->events[h->events_used++];
+    ev->l0_used = false;
+    return ev;
 }
 
 IntegrationEvents* next_integration_events(bh_hip_handle* h)
@@ -542,15 +552,19 @@ const char* lff_incompatible(const bh_hip_params_t& p)
 }
 
 // The walk(s) of one force evaluation, queued behind a build.
-void queue_walks(bh_hip_handle* h)
+void queue_walks(bh_hip_handle* h, StepEvents* ev)
 {
     if (lff_ready(h)) {
         bh::WalkParams wp = walk_params(h->p);
         wp.theta2 = h->lff_cfg.theta * h->lff_cfg.theta;
         wp.max_acc = 0.0f;  // clamped after the correction
         bh::launch_walk_record(h->d, wp, h->stats, h->lff, h->lff_cfg.band, h->lff_cfg.min_count);
+        if (ev) {
+            rec(ev->l0, h->d.stream);
+            ev->l0_used = true;
+        }
         bh::launch_lff_apply(h->d, h->lff, h->lff_cfg.band, wp.eps2, h->lff_cfg.min_count,
-                             h->p.max_acceleration);
+                             h->p.max_acceleration, h->stats);
     } else if (h->pm.N > 0) {
         bh::WalkParams wp = walk_params(h->p);
         wp.max_acc = 0.0f;  // clamped after the mesh force
@@ -572,7 +586,7 @@ bool queue_force(bh_hip_handle* h)
     StepEvents* ev = next_events(h);
     for (int attempt = 0;; ++attempt) {
         queue_build(h, ev);
-        queue_walks(h);
+        queue_walks(h, ev);
         const Built b = check_build(h);
         if (b == Built::kOk)
             break;
@@ -593,7 +607,7 @@ void begin_call(bh_hip_handle* h)
 {
     h->events_used = 0;
     h->int_used = 0;
-    bh::launch_fill_u32(reinterpret_cast<uint32_t*>(h->stats), 0u, 12, h->d.stream);
+    bh::launch_fill_u32(reinterpret_cast<uint32_t*>(h->stats), 0u, 16, h->d.stream);
 }
 
 // After a synchronised call: fold event times and counters into h->t.
@@ -615,6 +629,8 @@ void end_call(bh_hip_handle* h)
             span(e.p1, e.t1, t.tree_ms);
             span(e.t1, e.b1, t.moments_ms);
             span(e.b1, e.w1, t.walk_ms);
+            if (e.l0_used)
+                span(e.l0, e.w1, t.lff_ms);
         }
         for (size_t k = 0; k < h->int_used; ++k) {
             float a = 0.0f;
@@ -622,7 +638,7 @@ void end_call(bh_hip_handle* h)
                 t.integrate_ms += a;
         }
     }
-    unsigned long long st[6] = {0, 0, 0, 0, 0, 0};
+    unsigned long long st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (hipMemcpy(st, h->stats, sizeof(st), hipMemcpyDeviceToHost) == hipSuccess) {
         t.pp_interactions = st[0];
         t.pc_interactions = st[1];
@@ -630,6 +646,8 @@ void end_call(bh_hip_handle* h)
         t.far_interactions = st[3];
         t.far_evaluations = st[4];
         t.mono_interactions = st[5];
+        t.lff_pairs = st[6];
+        t.lff_tiles = st[7];
     }
     int info[2 * bh::kLevels + 4];
     if (hipMemcpy(info, h->d.level_info, sizeof(info), hipMemcpyDeviceToHost) == hipSuccess) {
@@ -920,7 +938,7 @@ void bh_hip_destroy(bh_hip_handle_t* h)
         (void)hipFree(p);
     h->allocations.clear();
     for (auto& e : h->events) {
-        for (hipEvent_t x : {e.b0, e.s0, e.s1, e.p1, e.t1, e.b1, e.w1})
+        for (hipEvent_t x : {e.b0, e.s0, e.s1, e.p1, e.t1, e.b1, e.w1, e.l0})
             if (x)
                 (void)hipEventDestroy(x);
     }
@@ -1432,22 +1450,30 @@ bool lff_alloc_core(bh_hip_handle* h)
         !dalloc(h, &L.tuple_counter, 2, "lff counter"))
         return false;
     short* frag = nullptr;
-    if (!dalloc(h, &frag, 24 * 64 * 4, "lff fragments"))
+    if (!dalloc(h, &frag, 14 * 64 * 8, "lff fragments"))  // 14 x 64 lanes x 8 BF16
         return false;
     L.frag = frag;
+    // the features carry the nodes' third moments: computed by every build
+    // from now on (the current tree has none)
+    if (!h->d.noct) {
+        if (!dalloc(h, &h->d.noct, 10 * h->d.node_cap, "node third moments"))
+            return false;
+        h->have_tree = false;
+    }
     // He-normal hidden weights, small output layer, zero biases (seeded).
     std::vector<float> p(bh::kLffParams, 0.0f);
     std::mt19937_64 rng(20261202ull);
     std::normal_distribution<float> nd(0.0f, 1.0f);
     const int NF = bh::kLffInputs, NH = bh::kLffHidden, NO = bh::kLffOutputs;
     for (int i = 0; i < NF * NH; ++i)
-        p[i] = nd(rng) * std::sqrt(2.0f / 15.0f);
+        p[i] = nd(rng) * std::sqrt(2.0f / 29.0f);
     for (int i = 0; i < NH * NH; ++i)
         p[NF * NH + NH + i] = nd(rng) * std::sqrt(2.0f / NH);
     for (int i = 0; i < NH * NO; ++i)
         p[NF * NH + NH + NH * NH + NH + i] = nd(rng) * 0.01f;
-    for (int k = 0; k < NH; ++k)  // the pad input carries no weight
-        p[(NF - 1) * NH + k] = 0.0f;
+    for (int f : {11, 26, 27})  // the pad inputs carry no weight
+        for (int k = 0; k < NH; ++k)
+            p[f * NH + k] = 0.0f;
     if (!check(h, hipMemcpy(L.params, p.data(), p.size() * 4, hipMemcpyHostToDevice), "lff init"))
         return false;
     bh::launch_lff_pack(h->d, L);

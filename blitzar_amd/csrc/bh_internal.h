@@ -26,12 +26,12 @@ constexpr int kTicketQueues = 8;
 constexpr int kTicketStride = 64;  // uint32 words between counters
 constexpr int kTicketWords = kTicketQueues * kTicketStride;
 
-// Learned far-field MLP: 16 inputs (15 features + pad) -> 64 -> 64 -> 3.
-constexpr int kLffInputs = 16;
+// Learned far-field MLP: 32 inputs (29 features + 3 pads) -> 64 -> 64 -> 3.
+constexpr int kLffInputs = 32;
 constexpr int kLffHidden = 64;
 constexpr int kLffOutputs = 3;
 constexpr int kLffParams = kLffInputs * kLffHidden + kLffHidden + kLffHidden * kLffHidden +
-                           kLffHidden + kLffHidden * kLffOutputs + kLffOutputs;  // 5443
+                           kLffHidden + kLffHidden * kLffOutputs + kLffOutputs;  // 6467
 
 // Root geometry, written on the device once per build by geometry_kernel.
 //   FP32 part: the quantisation frame of the key kernel (AABB of the
@@ -192,6 +192,8 @@ struct Device {
     uint32_t* nparent = nullptr;  // parent index (root: 0xffffffff)
     WalkNode* wnode = nullptr;
     float* next = nullptr;            // per node: particle extent about the lattice centre
+    float* noct = nullptr;            // per node: 10 third moments about the COM (learned
+                                      // correction only; null otherwise)
     uint32_t* gscan = nullptr;        // per-node group counts -> offsets
     uint32_t* gscan_blocks = nullptr;
     uint4* groups = nullptr;          // walk groups: first particle, count, node
@@ -239,9 +241,9 @@ struct LffState {
     float* loss_partial = nullptr;
     float* loss_acc = nullptr;
     int partial_blocks = 0;
-    void* frag = nullptr;         // 24 x 64 BF16x4 MFMA fragments
+    void* frag = nullptr;         // 14 x 64 BF16x8 MFMA fragments
     float* bias = nullptr;        // b1[64], b2[64], b3[16]
-    float* X = nullptr;           // training features, tuple_cap x 16
+    float* X = nullptr;           // training features, tuple_cap x kLffInputs
     float* Y = nullptr;           // training targets, tuple_cap x 3
     unsigned int* tuple_counter = nullptr;  // [0] tuples held, [1] scratch count
     size_t tuple_cap = 0;
@@ -323,7 +325,7 @@ void launch_lff_adam(Device& d, const LffState& L, int nblocks, float* P, float*
 void launch_lff_pack(Device& d, const LffState& L);
 void launch_lff_forward(Device& d, const LffState& L, const float* X, uint32_t n, float* Y);
 void launch_lff_apply(Device& d, const LffState& L, float band, float eps2, int min_count,
-                      float max_acc);
+                      float max_acc, unsigned long long* stats);
 // k_integrate.hip
 void launch_kick(Device& d, double dt_kick, double max_speed);
 // box > 0: positions wrapped into [0, box) after the drift

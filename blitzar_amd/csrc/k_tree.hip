@@ -341,9 +341,64 @@ __global__ __launch_bounds__(256) void tree_emit_kernel(
     }
 }
 
+// Third moments about the COM, T_ijk = sum m y_i y_j y_k (10 components:
+// xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz), accumulated in registers.
+struct Oct {
+    float t[10];
+};
+
+__device__ inline void oct_zero(Oct& o)
+{
+#pragma unroll
+    for (int k = 0; k < 10; ++k)
+        o.t[k] = 0.0f;
+}
+
+// o += m y y y (one particle at offset y from the COM)
+__device__ inline void oct_add_point(Oct& o, float m, float x, float y, float z)
+{
+    const float mx = m * x, my = m * y, mz = m * z;
+    o.t[0] += mx * x * x;
+    o.t[1] += mx * x * y;
+    o.t[2] += mx * x * z;
+    o.t[3] += mx * y * y;
+    o.t[4] += mx * y * z;
+    o.t[5] += mx * z * z;
+    o.t[6] += my * y * y;
+    o.t[7] += my * y * z;
+    o.t[8] += my * z * z;
+    o.t[9] += mz * z * z;
+}
+
+// o += the third moment of a child about the parent COM (parallel-axis
+// shift): T_c + (D_i S_jk + D_j S_ik + D_k S_ij) + M_c D_i D_j D_k, with D
+// the child COM minus the parent COM and S = (Q + trs I) / 3 the child's
+// second moment about its own COM (Q traceless).
+__device__ inline void oct_add_child(Oct& o, const float* tc, const WalkNode& c, float dx,
+                                     float dy, float dz)
+{
+    const float third = 1.0f / 3.0f;
+    const float sxx = (c.qxx + c.trs) * third, syy = (c.qyy + c.trs) * third;
+    const float szz = (-c.qxx - c.qyy + c.trs) * third;
+    const float sxy = c.qxy * third, sxz = c.qxz * third, syz = c.qyz * third;
+    const float D[3] = {dx, dy, dz};
+    const float S[3][3] = {{sxx, sxy, sxz}, {sxy, syy, syz}, {sxz, syz, szz}};
+    const int I[10] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 2};
+    const int J[10] = {0, 0, 0, 1, 1, 2, 1, 1, 2, 2};
+    const int K[10] = {0, 1, 2, 1, 2, 2, 1, 2, 2, 2};
+#pragma unroll
+    for (int k = 0; k < 10; ++k) {
+        const int i = I[k], j = J[k], l = K[k];
+        o.t[k] += tc[k] + D[i] * S[j][l] + D[j] * S[i][l] + D[l] * S[i][j] +
+                  c.mass * D[i] * D[j] * D[l];
+    }
+}
+
 // Moments of one level (deepest level first). Leaves also pack their
 // particles: spart[s] = (FP32 offset from the leaf lattice centre, mass) and
-// sleaf[s] = leaf node.
+// sleaf[s] = leaf node. OCT: also the third moments (noct, 10 per node; read
+// by the learned far-field correction).
+template <bool OCT>
 __global__ __launch_bounds__(256) void tree_moments_kernel(
     const uint64_t* __restrict__ nprefix, const uint32_t* __restrict__ nstart,
     const uint32_t* __restrict__ ncount, const uint32_t* __restrict__ nchild,
@@ -351,7 +406,7 @@ __global__ __launch_bounds__(256) void tree_moments_kernel(
     const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
     const float* __restrict__ pm, const RootGeom* __restrict__ geom, WalkNode* __restrict__ wn,
     float4* __restrict__ spart, uint32_t* __restrict__ sleaf, float* __restrict__ ext,
-    int refit, int* err)
+    float* __restrict__ noct, int refit, int* err)
 {
     const int base = info[level];
     const int cnt = info[kLevels + level];
@@ -369,6 +424,9 @@ __global__ __launch_bounds__(256) void tree_moments_kernel(
         // cell's own half-width 2^(21 - level) unless they left the cell)
         float e = 0.0f;
         uint32_t count = 0;
+        Oct oc;
+        if (OCT)
+            oct_zero(oc);
         if (nc == 0) {
             count = ncount[node];
             const double ox = geom->lo_d[0] + static_cast<double>(c[0]) * geom->unit_d[0];
@@ -406,6 +464,8 @@ __global__ __launch_bounds__(256) void tree_moments_kernel(
                 qyz += p.w * (3.0f * dy * dz);
                 trs += p.w * r2;
                 bmax = fmaxf(bmax, r2);
+                if (OCT)
+                    oct_add_point(oc, p.w, dx, dy, dz);
             }
             bmax = sqrtf(bmax);
         } else {
@@ -449,10 +509,19 @@ __global__ __launch_bounds__(256) void tree_moments_kernel(
                 if (live) {
                     const int off = max(abs(ch.cx - c[0]), max(abs(ch.cy - c[1]), abs(ch.cz - c[2])));
                     e = fmaxf(e, ext[first + jj] + static_cast<float>(off));
+                    if (OCT)
+                        oct_add_child(oc, noct + static_cast<size_t>(first + jj) * 10, ch, dx, dy,
+                                      dz);
                 }
             }
         }
         ext[node] = e;
+        if (OCT) {
+            float* o = noct + static_cast<size_t>(node) * 10;
+#pragma unroll
+            for (int k = 0; k < 10; ++k)
+                o[k] = oc.t[k];
+        }
         // Refit (moments of the current positions on an earlier topology):
         // particles may have left their cells, so the level the walk derives
         // the cell size from is lowered until that cell's half-width covers
@@ -662,10 +731,16 @@ void launch_tree_moments(Device& d, bool refit)
 {
     for (int L = kMaxLevel; L >= 0; --L) {
         const size_t bound = level_bound(L, d.n, d.node_cap);
-        tree_moments_kernel<<<level_grid(bound), 256, 0, d.stream>>>(
-            d.nprefix, d.nstart, d.ncount, d.nchild, d.nfirst, d.level_info, L, d.pos[0],
-            d.pos[1], d.pos[2], d.mass, d.geom, d.wnode, d.spart, d.sleaf, d.next,
-            refit ? 1 : 0, d.err);
+        if (d.noct)
+            tree_moments_kernel<true><<<level_grid(bound), 256, 0, d.stream>>>(
+                d.nprefix, d.nstart, d.ncount, d.nchild, d.nfirst, d.level_info, L, d.pos[0],
+                d.pos[1], d.pos[2], d.mass, d.geom, d.wnode, d.spart, d.sleaf, d.next, d.noct,
+                refit ? 1 : 0, d.err);
+        else
+            tree_moments_kernel<false><<<level_grid(bound), 256, 0, d.stream>>>(
+                d.nprefix, d.nstart, d.ncount, d.nchild, d.nfirst, d.level_info, L, d.pos[0],
+                d.pos[1], d.pos[2], d.mass, d.geom, d.wnode, d.spart, d.sleaf, d.next, nullptr,
+                refit ? 1 : 0, d.err);
     }
 }
 

@@ -142,9 +142,44 @@ __device__ inline void eval_cells_dj(const float4* __restrict__ ca, const float4
         const float ri2 = ri * ri;
         const float ri3 = ri * ri2;
         float wgt = 1.0f;
-        if (TPM)
+        if (TPM && ORDER < 2)
             wgt = tpm_weight(r2 * ri, inv_rs);
-        if (ORDER >= 2) {
+        if (TPM && ORDER >= 2) {
+            // Second-order expansion of the short-range kernel g(R) d,
+            // g = w(R) / R^3, about the COM (S = (Q + trs I) / 3 the raw
+            // second moment, Q traceless):
+            //   a = [M g + G2 (dQd + trs |d|^2) / 6 + 5/6 G1 trs] d + G1 / 3 Q d
+            // with G1 = g'(R) / R, G2 = G1'(R) / R. For the erfc weight,
+            // w'(R) = -R^2 E / (2 sqrt(pi) rs^3), E = exp(-R^2 / 4 rs^2), so
+            //   G1 = -c / R^2 - 3 w / R^5,
+            //   G2 = c (1 / (2 rs^2 R^2) + 5 / R^4) + 15 w / R^7,
+            // c = E / (2 sqrt(pi) rs^3). With w = 1, c = 0 it is the softened
+            // Newtonian expansion of the open walk.
+            const float4 b = cb[j];
+            const float2 c = cc[j];  // (qyz, trs)
+            const float qx = fmaf(b.z, dz, fmaf(b.y, dy, b.x * dx));
+            const float qy = fmaf(c.x, dz, fmaf(b.w, dy, b.y * dx));
+            const float qz = fmaf(-(b.x + b.w), dz, fmaf(c.x, dy, b.z * dx));
+            const float dqd = fmaf(dz, qz, fmaf(dy, qy, dx * qx));
+            const float R = r2 * ri;
+            const float ha = 0.5f * R * inv_rs;
+            const float E = __expf(-ha * ha);
+            const float w = erfcf(ha) + R * inv_rs * 0.5641895835477563f * E;
+            const float cE = E * (0.28209479177387814f * inv_rs * inv_rs * inv_rs);
+            const float ri4 = ri2 * ri2;
+            const float wr5 = w * ri4 * ri;
+            const float g = w * ri3;
+            const float G1 = fmaf(-cE, ri2, -3.0f * wr5);
+            const float G2 = fmaf(cE, fmaf(0.5f * inv_rs * inv_rs, ri2, 5.0f * ri4),
+                                  15.0f * wr5 * ri2);
+            const float d2 = r2 - P.eps2;
+            const float s = fmaf(G2 * (1.0f / 6.0f), fmaf(c.y, d2, dqd),
+                                 fmaf(a.w, g, (5.0f / 6.0f) * G1 * c.y));
+            const float t = G1 * (1.0f / 3.0f);
+            ax = fmaf(t, qx, fmaf(s, dx, ax));
+            ay = fmaf(t, qy, fmaf(s, dy, ay));
+            az = fmaf(t, qz, fmaf(s, dz, az));
+        } else if (ORDER >= 2) {
             const float4 b = cb[j];
             const float2 c = cc[j];
             // b = (qxx, qxy, qxz, qyy), c = (qyz, eps^2 trs); qzz = -qxx - qyy
@@ -586,8 +621,9 @@ __global__ __launch_bounds__(64, FAR ? 1 : BH_WALK_MIN_WAVES) void walk_kernel(
                     if (REC && ok) {
                         // near the group: small nodes are opened, large ones
                         // accepted and recorded for the learned correction
+                        // (a group beyond the lists' capacity opens them all)
                         near = size * size >= rec_band2 * d2;
-                        if (near && meta_count(w.meta) < rec_min_count) {
+                        if (near && (meta_count(w.meta) < rec_min_count || g >= rec_groups)) {
                             ok = false;
                             near = false;
                         }
@@ -604,11 +640,20 @@ __global__ __launch_bounds__(64, FAR ? 1 : BH_WALK_MIN_WAVES) void walk_kernel(
             if (REC) {
                 near = near && accept;
                 const unsigned long long rm = __ballot(near);
-                if (near && g < rec_groups) {
+                if (near) {
                     const int slot = nrec + static_cast<int>(__popcll(rm & lt));
-                    if (slot < rec_cap)
+                    if (slot < rec_cap) {
                         rec_lists[static_cast<size_t>(g) * rec_cap + slot] =
                             static_cast<uint32_t>(node);
+                    } else {
+                        // the group's list is full: open the node instead
+                        // (its error is then the plain walk's, never an
+                        // uncorrected near node's)
+                        const uint32_t nch = meta_nchild(w.meta);
+                        accept = false;
+                        direct = nch == 0;
+                        open = nch != 0;
+                    }
                 }
                 nrec += static_cast<int>(__popcll(rm));
             }
@@ -659,7 +704,8 @@ __global__ __launch_bounds__(64, FAR ? 1 : BH_WALK_MIN_WAVES) void walk_kernel(
                 const int slot = ncell + static_cast<int>(__popcll(am & lt));
                 cA[slot] = make_float4(px, py, pz, w.mass);
                 cB[slot] = make_float4(w.qxx, w.qxy, w.qxz, w.qyy);
-                cC[slot] = make_float2(w.qyz, P.eps2 * w.trs);
+                // the open walk's expansion needs eps^2 trs, the TreePM one trs
+                cC[slot] = make_float2(w.qyz, TPM ? w.trs : P.eps2 * w.trs);
             }
             ncell += static_cast<int>(__popcll(am));
 

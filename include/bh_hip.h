@@ -154,6 +154,9 @@ typedef struct bh_hip_timings {
     double permute_ms;  /* equal-key tie order + store permutation */
     double tree_ms;     /* tree levels (topology) */
     double moments_ms;  /* moments, leaf packing and walk groups */
+    uint64_t lff_pairs; /* learned correction: (target, node) pairs evaluated */
+    uint64_t lff_tiles; /* learned correction: 16-pair MFMA tiles run */
+    double lff_ms;      /* learned correction: the MLP pass (part of walk_ms) */
 } bh_hip_timings_t;
 
 /* Fill `out` with the defaults (reference config defaults, CfgConfig.hpp). */
@@ -244,10 +247,11 @@ bh_hip_result_t bh_hip_get_timings(bh_hip_handle_t* h, bh_hip_timings_t* out);
 bh_hip_result_t bh_hip_get_last_error(bh_hip_handle_t* h, char* buf, size_t cap);
 
 /* ---- Learned far-field correction --------------------------------------
- * No counterpart in the reference. A small MLP (16 -> 64 -> 64 -> 3, SiLU,
+ * No counterpart in the reference. A small MLP (32 -> 64 -> 64 -> 3, SiLU,
  * BF16 MFMA inference with FP32 accumulation) predicts, for an accepted node
  * of at least min_count particles close to a target (size / distance >=
- * band), the residual of the second-order expansion in units of M / r^2; the
+ * band), the residual of the second-order expansion in units of M / r^2 from
+ * the node's moments up to third order and the target-node geometry; the
  * corrected walk runs at a larger opening angle. Training tuples come from this solver's own walks
  * (exact residual = direct sum over the node's particles) and the network is
  * trained on the GPU by this library's forward/backward kernels (Adam). */
@@ -260,8 +264,8 @@ typedef struct bh_hip_lff_config {
                           the angle `band`) instead of corrected (default 32) */
 } bh_hip_lff_config_t;
 
-#define BH_HIP_LFF_INPUTS 16
-#define BH_HIP_LFF_PARAMS 5443 /* W1[16][64] b1[64] W2[64][64] b2[64] W3[64][3] b3[3] */
+#define BH_HIP_LFF_INPUTS 32
+#define BH_HIP_LFF_PARAMS 6467 /* W1[32][64] b1[64] W2[64][64] b2[64] W3[64][3] b3[3] */
 
 bh_hip_result_t bh_hip_lff_configure(bh_hip_handle_t* h, const bh_hip_lff_config_t* cfg);
 /* Build the tree at the current positions, walk at cfg->theta recording the
@@ -278,7 +282,7 @@ bh_hip_result_t bh_hip_lff_train(bh_hip_handle_t* h, int32_t epochs, int32_t bat
 bh_hip_result_t bh_hip_lff_get_params(bh_hip_handle_t* h, float* params);
 bh_hip_result_t bh_hip_lff_set_params(bh_hip_handle_t* h, const float* params);
 /* Gradient of the mean over n samples of |MLP(x) - y|^2 with the given
- * parameters (FP32 training kernels); X is n x 16, Y is n x 3. */
+ * parameters (FP32 training kernels); X is n x 32, Y is n x 3. */
 bh_hip_result_t bh_hip_lff_gradient(bh_hip_handle_t* h, const float* X, const float* Y, uint32_t n,
                                     const float* params, float* grad, float* loss);
 /* MLP outputs for n feature rows through the BF16 MFMA inference kernel. */

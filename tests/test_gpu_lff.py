@@ -7,7 +7,7 @@ Tolerances:
     model: 99% of outputs within 1e-5 * RMS(output), all within 2e-2 * RMS
     (an occasional one-ulp BF16 flip of a hidden value); vs FP64: 5e-2 * RMS
   * training tuples vs FP64 features / residuals of the same (target, node):
-    features within 1e-3, residuals within 1e-3 + 1e-2 |y|
+    features within 1e-3 (1 + |f|), residuals within 1e-3 + 1e-2 |y|
   * an all-zero network leaves the coarse walk's forces bit-identical
   * trained on two Plummer realisations and applied to a third: the held-out
     mean-square residual of the corrected nodes drops by >= 20% (median by
@@ -88,8 +88,9 @@ def test_zero_network_leaves_coarse_walk_unchanged():
     with B.OctreeHip(n, theta=0.9, softening=0.01) as s:
         s.upload(pos, vel, m)
         a0 = s.compute_accelerations()
-        # min_count = 0: no near node is opened, so the walk is the plain one
-        s.lff_configure(theta=0.9, band=0.5, enabled=True, min_count=0)
+        # min_count = 0 and lists long enough for every near node: no near
+        # node is opened, so the walk is the plain one
+        s.lff_configure(theta=0.9, band=0.5, enabled=True, min_count=0, list_cap=4096)
         s.lff_set_params(np.zeros(mlp.NP, np.float32))
         a1 = s.compute_accelerations()
         s.lff_enable(False)
@@ -116,42 +117,46 @@ def test_tuples_match_fp64_oracle():
         t = s.dump_tree()
     assert total == len(X) > 1000
     assert np.all(np.isfinite(X)) and np.all(np.isfinite(Y))
-    assert np.all((X[:, 15] > 0.0) & (X[:, 15] <= 1.0))  # eps^2 / (d^2 + eps^2)
-    assert np.all(X[:, 10] <= 1.0 / band * (1 + 1e-5))  # size >= band * distance
+    assert np.all((X[:, 28] > 0.0) & (X[:, 28] <= 1.0))  # eps^2 / (d^2 + eps^2)
+    assert np.all(X[:, 15] <= 1.0 / band * (1 + 1e-5))  # size >= band * distance
+    assert np.all(X[:, list(mlp.PADS)] == 0.0)
     g = t["geom"]
     lo, unit = g[6:9], g[9:12]
     mo = t["moments"]
     ids = t["sorted_ids"]
     nn = len(t["level"])
     m64 = m.astype(np.float64)
-    sig = np.zeros((nn, 11))
+    sig_cols = [0, 7, 8, 9, 10, 1, 2, 3, 4, 5, 6]  # count, COM offset, bmax, quadrupole
+    sig = np.zeros((nn, len(sig_cols)))
     node = []
     for i in range(nn):
         mem = ids[t["start"][i]:t["start"][i] + t["count"][i]]
         M, com, q, _ = mlp.node_moments(pos[mem], m64[mem])
+        o = mlp.node_third_moments(pos[mem], m64[mem])
         centre = lo + t["lattice"][i].astype(np.float64) * unit
         f = mlp.node_features(com, com, M, q, float(mo[i, 10]), centre, float(mo[i, 9]),
-                              int(t["count"][i]), eps)[0]
-        sig[i] = f[[0, 11, 12, 13, 14, 1, 2, 3, 4, 5, 6]]
-        node.append((mem, com, M, q, centre))
-    dn, inode = cKDTree(sig).query(X[:, [0, 11, 12, 13, 14, 1, 2, 3, 4, 5, 6]].astype(np.float64))
+                              int(t["count"][i]), eps, o)[0]
+        sig[i] = f[sig_cols]
+        node.append((mem, com, M, q, centre, o))
+    dn, inode = cKDTree(sig).query(X[:, sig_cols].astype(np.float64))
     assert dn.max() < 1e-3
     ptree = cKDTree(pos)
     seen = set()
     ferr, yerr = [], []
     for k in range(len(X)):
         i = inode[k]
-        mem, com, M, q, centre = node[i]
+        mem, com, M, q, centre, o = node[i]
         size = float(mo[i, 10])
-        x_est = com + size * X[k, 7:10].astype(np.float64)
+        x_est = com + size * X[k, 12:15].astype(np.float64)
         dp, j = ptree.query(x_est)
         assert dp <= 1e-3 * size + 1e-6, (k, i, dp, size)
         assert (j, i) not in seen
         seen.add((j, i))
         assert len(mem) >= 8  # smaller near nodes are opened, not corrected
-        f = mlp.node_features(pos[j], com, M, q, size, centre, float(mo[i, 9]), len(mem), eps)[0]
+        f = mlp.node_features(pos[j], com, M, q, size, centre, float(mo[i, 9]), len(mem), eps,
+                              o)[0]
         y = mlp.node_residual(pos[j], pos[mem], m64[mem], eps)[0]
-        ferr.append(np.abs(X[k] - f).max())
+        ferr.append((np.abs(X[k] - f) / (1.0 + np.abs(f))).max())
         yerr.append(np.abs(Y[k] - y).max() / (1e-3 + 1e-2 * np.abs(y).max()))
     say(f"tuples={len(X)} max feature err={max(ferr):.2e} max y err/tol={max(yerr):.2e}")
     assert max(ferr) <= 1e-3

@@ -76,7 +76,7 @@ def test_bf16_rounding():
 
 def test_bf16_forward_within_rounding_of_fp64():
     rng = np.random.default_rng(8)
-    p = _params(rng, 0.2)
+    p = _params(rng, 0.15)
     X = rng.normal(size=(500, mlp.NF))
     y = mlp.forward(p, X)
     yb = mlp.forward_bf16(p, X)
@@ -125,21 +125,89 @@ def test_features_and_residual_invariances():
     m = rng.uniform(0.5, 1.5, 25)
     x = np.array([[1.2, 0.4, -0.3], [0.1, 1.0, 0.9]])
     M, com, q, _ = mlp.node_moments(pos, m)
+    o = mlp.node_third_moments(pos, m)
     centre = np.zeros(3)
-    f = mlp.node_features(x, com, M, q, 1.0, centre, 0.8, 25, 0.02)
+    f = mlp.node_features(x, com, M, q, 1.0, centre, 0.8, 25, 0.02, o)
     r = mlp.node_residual(x, pos, m, 0.02)
     k, c, t = 3.7, 0.25, np.array([5.0, -2.0, 1.0])
     M2, com2, q2, _ = mlp.node_moments(k * pos + t, c * m)
-    f2 = mlp.node_features(k * x + t, com2, M2, q2, k, k * centre + t, k * 0.8, 25, k * 0.02)
+    o2 = mlp.node_third_moments(k * pos + t, c * m)
+    f2 = mlp.node_features(k * x + t, com2, M2, q2, k, k * centre + t, k * 0.8, 25, k * 0.02, o2)
     r2 = mlp.node_residual(k * x + t, k * pos + t, c * m, k * 0.02)
     np.testing.assert_allclose(f2, f, rtol=1e-10, atol=1e-12)
     np.testing.assert_allclose(r2, r, rtol=1e-9, atol=1e-13)
-    # f10 = |f7..f9|; f6 = -(f1 + f4) (traceless); f0 = log2(count)/16
-    np.testing.assert_allclose(f[:, 10], np.linalg.norm(f[:, 7:10], axis=1), rtol=1e-14)
+    # f15 = |f12..f14|; f6 = -(f1 + f4) (traceless); f0 = log2(count)/16; pads
+    np.testing.assert_allclose(f[:, 15], np.linalg.norm(f[:, 12:15], axis=1), rtol=1e-14)
     np.testing.assert_allclose(f[:, 6], -(f[:, 1] + f[:, 4]), rtol=1e-12, atol=1e-15)
     assert f[0, 0] == pytest.approx(np.log2(25) / 16)
+    assert np.all(f[:, list(mlp.PADS)] == 0.0)
     d2 = np.sum((x - com) ** 2, axis=1)
-    np.testing.assert_allclose(f[:, 15], 0.02 ** 2 / (d2 + 0.02 ** 2), rtol=1e-14)
+    np.testing.assert_allclose(f[:, 28], 0.02 ** 2 / (d2 + 0.02 ** 2), rtol=1e-14)
+    # the octupole features are the expansion's third-order term: at a
+    # distant target the second-order residual is that term to within
+    # (size / distance)^4
+    far = com + 40.0 * np.array([[0.36, 0.48, 0.8]])
+    ff = mlp.node_features(far, com, M, q, 1.0, centre, 0.8, 25, 0.0, o)[0]
+    rf = mlp.node_residual(far, pos, m, 0.0)[0]
+    assert np.linalg.norm(rf - ff[29:32]) < 0.05 * np.linalg.norm(rf)
+
+
+def test_third_moments_two_particle_closed_form_and_parallel_axis():
+    # m1 at +a1 e_z, m2 at -a2 e_z with m1 a1 = m2 a2 (COM at the origin):
+    # T_zzz = m1 a1^3 - m2 a2^3, every other component zero
+    m1, a1, a2 = 1.5, 0.4, 0.6
+    m2 = m1 * a1 / a2
+    o = mlp.node_third_moments(np.array([[0, 0, a1], [0, 0, -a2]]), np.array([m1, m2]))
+    want = np.zeros(10)
+    want[9] = m1 * a1 ** 3 - m2 * a2 ** 3
+    np.testing.assert_allclose(o, want, atol=1e-15)
+    # parallel-axis shift of two children (what the tree's upward pass does)
+    rng = np.random.default_rng(9)
+    pa = rng.normal(size=(30, 3)) * 0.2 + [0.3, 0.1, -0.2]
+    pb = rng.normal(size=(20, 3)) * 0.1 + [-0.4, 0.2, 0.5]
+    ma, mb = rng.uniform(0.5, 1.5, 30), rng.uniform(0.5, 1.5, 20)
+    allp, allm = np.vstack([pa, pb]), np.concatenate([ma, mb])
+    _, com, _, _ = mlp.node_moments(allp, allm)
+    shifted = np.zeros(10)
+    for p_, m_ in ((pa, ma), (pb, mb)):
+        Mc, cc, qc, trs = mlp.node_moments(p_, m_)
+        S = (np.array([[qc[0], qc[1], qc[2]], [qc[1], qc[3], qc[4]], [qc[2], qc[4], qc[5]]])
+             + trs * np.eye(3)) / 3.0
+        D = cc - com
+        Tc = mlp.node_third_moments(p_, m_)
+        for k, (i, j, l) in enumerate(mlp._OCT_IDX):
+            shifted[k] += (Tc[k] + D[i] * S[j, l] + D[j] * S[i, l] + D[l] * S[i, j]
+                           + Mc * D[i] * D[j] * D[l])
+    np.testing.assert_allclose(shifted, mlp.node_third_moments(allp, allm), rtol=1e-12,
+                               atol=1e-14)
+
+
+@pytest.mark.parametrize("u", [0.2, 0.05])
+def test_octupole_term_on_axis_and_order(u):
+    # on the axis of the pair above the u^3 term of -sum m / (D -+ a)^2 is
+    # -4 T_zzz / D^5
+    m1, a1, a2 = 1.5, 0.4, 0.6
+    m2 = m1 * a1 / a2
+    pos = np.array([[0, 0, a1], [0, 0, -a2]])
+    o = mlp.node_third_moments(pos, np.array([m1, m2]))
+    D = a2 / u
+    got = mlp.octupole_accel(np.array([[0, 0, D]]), np.zeros(3), o)[0]
+    assert got[2] == pytest.approx(-4 * o[9] / D ** 5, rel=1e-12)
+    assert abs(got[0]) < 1e-15 and abs(got[1]) < 1e-15
+    # a random node: (second-order residual - octupole term) is fourth order
+    rng = np.random.default_rng(11)
+    p = rng.uniform(-0.5, 0.5, (40, 3))
+    m = rng.uniform(0.5, 1.5, 40)
+    M, com, _, _ = mlp.node_moments(p, m)
+    oo = mlp.node_third_moments(p, m)
+    dirn = np.array([0.6, -0.48, 0.64])
+    rest = []
+    for dist in (30.0, 60.0):
+        x = com + dist * dirn
+        mono = M / dist ** 2
+        res = mlp.node_residual(x, p, m, 0.0)[0] * mono
+        rest.append(np.linalg.norm(res - mlp.octupole_accel(x, com, oo)[0]) / mono)
+    assert 12.0 < rest[0] / rest[1] < 20.0
 
 
 @pytest.mark.parametrize("eps", [0.0, 0.3, 1.0, 4.0])
