@@ -98,6 +98,7 @@ EXPORTS = [
     "bh_hip_resolve_params", "bh_hip_get_params",
     "bh_hip_direct_forces_fp64", "bh_hip_diagnostics", "bh_hip_set_timing",
     "bh_hip_get_timings", "bh_hip_get_last_error", "bh_hip_sort_pairs",
+    "bh_hip_sort_pairs_noalloc",
     "bh_hip_lff_configure", "bh_hip_lff_collect", "bh_hip_lff_train", "bh_hip_lff_get_params",
     "bh_hip_lff_set_params", "bh_hip_lff_gradient", "bh_hip_lff_forward",
     "bh_hip_lff_get_tuples", "bh_hip_adaptive_levels", "bh_hip_pm_accelerations",
@@ -181,6 +182,11 @@ def lib() -> C.CDLL:
     L.bh_hip_sort_pairs.argtypes = [_p(np.uint64), _p(np.uint32), C.c_uint64, C.c_int32,
                                     _p(np.uint64), _p(np.uint32), C.c_char_p, C.c_size_t]
     L.bh_hip_sort_pairs.restype = R
+    # device pointers and a stream handle, passed as plain addresses
+    L.bh_hip_sort_pairs_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int32,
+                                            C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p,
+                                            C.c_size_t]
+    L.bh_hip_sort_pairs_noalloc.restype = R
     L.bh_hip_lff_configure.argtypes = [H, C.POINTER(LffConfig)]
     L.bh_hip_lff_collect.argtypes = [H, C.c_int32, C.c_uint64, C.POINTER(C.c_uint64)]
     L.bh_hip_lff_train.argtypes = [H, C.c_int32, C.c_int32, C.c_float, C.c_uint64,

@@ -278,6 +278,13 @@ size_t radix_hist_words(size_t n);
 int radix_sort_passes(int key_bits);
 bool radix_sort_buffers(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt,
                         uint32_t* hist_ws, size_t n, int key_bits, int* err, hipStream_t stream);
+// The same sort reading (keys_in, vals_in), which it never writes: pass 0
+// writes buffer pair a, pass 1 pair b, and so on alternately. Returns true
+// when the result ends in pair a (an odd number of passes), false when it
+// ends in pair b. Nothing is allocated.
+bool radix_sort_from(const uint64_t* keys_in, const uint32_t* vals_in, uint64_t* keys_a,
+                     uint32_t* vals_a, uint64_t* keys_b, uint32_t* vals_b, uint32_t* hist_ws,
+                     size_t n, int key_bits, int* err, hipStream_t stream);
 // Exclusive scan of `data` in place (one launch). The element count is *n_dev
 // when n_dev is non-null (then max_n bounds it), else n_host. The total is
 // written to total_out / total_out_i when non-null. `block_sums` is the scan's

@@ -275,8 +275,9 @@ size_t radix_hist_words(size_t n)
 
 int radix_sort_passes(int key_bits) { return (key_bits + 7) / 8; }
 
-bool radix_sort_buffers(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt,
-                        uint32_t* hist_ws, size_t n, int key_bits, int* err, hipStream_t stream)
+bool radix_sort_from(const uint64_t* keys_in, const uint32_t* vals_in, uint64_t* keys_a,
+                     uint32_t* vals_a, uint64_t* keys_b, uint32_t* vals_b, uint32_t* hist_ws,
+                     size_t n, int key_bits, int* err, hipStream_t stream)
 {
     const int passes = radix_sort_passes(key_bits);
     if (n <= 1)
@@ -285,10 +286,12 @@ bool radix_sort_buffers(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint
     const size_t hn = static_cast<size_t>(ntiles) * 256;
     uint32_t* hist = hist_ws;
     uint32_t* sums = hist_ws + hn;
-    unsigned long long* kin = reinterpret_cast<unsigned long long*>(keys);
-    unsigned long long* kout = reinterpret_cast<unsigned long long*>(keys_alt);
-    uint32_t* vin = vals;
-    uint32_t* vout = vals_alt;
+    const unsigned long long* kin = reinterpret_cast<const unsigned long long*>(keys_in);
+    const uint32_t* vin = vals_in;
+    unsigned long long* kout = reinterpret_cast<unsigned long long*>(keys_a);
+    uint32_t* vout = vals_a;
+    unsigned long long* knext = reinterpret_cast<unsigned long long*>(keys_b);
+    uint32_t* vnext = vals_b;
     for (int p = 0; p < passes; ++p) {
         const int shift = 8 * p;
         // The last digit covers only the remaining key bits.
@@ -298,10 +301,21 @@ bool radix_sort_buffers(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint
         launch_scan_exclusive(hist, hn, nullptr, hn, sums, nullptr, nullptr, err, stream);
         scatter_kernel<<<ntiles, kBlock, 0, stream>>>(kin, vin, kout, vout, n, shift, dmask,
                                                       hist, ntiles);
-        std::swap(kin, kout);
-        std::swap(vin, vout);
+        // this pass's output is the next pass's input; the next output is the
+        // other buffer pair
+        kin = kout;
+        vin = vout;
+        std::swap(kout, knext);
+        std::swap(vout, vnext);
     }
     return (passes & 1) != 0;
+}
+
+bool radix_sort_buffers(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt,
+                        uint32_t* hist_ws, size_t n, int key_bits, int* err, hipStream_t stream)
+{
+    return radix_sort_from(keys, vals, keys_alt, vals_alt, keys, vals, hist_ws, n, key_bits, err,
+                           stream);
 }
 
 void launch_radix_sort(Device& d, size_t n, int key_bits)

@@ -310,6 +310,22 @@ bh_hip_result_t bh_hip_sort_pairs(const uint64_t* keys, const uint32_t* vals, ui
                                   int32_t key_bits, uint64_t* keys_out, uint32_t* vals_out,
                                   char* err, size_t err_cap);
 
+/* Allocation-free variant of bh_hip_sort_pairs. It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * keys / vals / keys_out / vals_out are DEVICE pointers of the current device;
+ * each output array is either its input array (in place) or disjoint from
+ * every other array. stream is a hipStream_t (NULL: the null stream); the
+ * call waits for it before returning. Same ordering as bh_hip_sort_pairs
+ * (stable, by the low key_bits bits). Scratch lives in a fixed device array
+ * (about 24.6 MiB per GPU, held until the process ends): n above 2,097,152
+ * pairs is rejected with BH_HIP_INVALID_ARGUMENT. Calls that overlap on one
+ * GPU share that array and race. */
+bh_hip_result_t bh_hip_sort_pairs_noalloc(const uint64_t* keys, const uint32_t* vals, uint64_t n,
+                                          int32_t key_bits, uint64_t* keys_out,
+                                          uint32_t* vals_out, void* stream, char* err,
+                                          size_t err_cap);
+
 /* Library version string. */
 const char* bh_hip_version(void);
 
