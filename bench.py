@@ -106,9 +106,10 @@ PMC_SUMMARY = os.path.join(ROOT, "profiles", "walk_build_merger8M_relative0.015_
 
 
 def walk_traffic(args, n):
-    """HBM bytes per walk launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of this workload (2 x FETCH + WRITE, calibrated on the
-    streaming kick/drift kernel), or None for other workloads."""
+    """Bytes per walk launch between L2 and the memory fabric, from the
+    committed rocprofv3 summary of this workload (raw read / write request
+    counters; 128 B per read request and 64 B per 64-byte write request, as a
+    1 GiB copy in the same run measures them), or None for other workloads."""
     if not (args.workload == "merger" and n == 8388608 and args.order == 2 and args.bucket == 16
             and args.criterion == "relative" and args.theta == 2.0
             and args.relative_alpha == 0.015 and args.relative_mono == 0.1
@@ -417,7 +418,7 @@ def main(argv=None):
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device-to-device copy of MIB MiB three times "
                          "(reads and writes MIB MiB each): the known-size kernel that "
-                         "calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
+                         "calibrates rocprofv3's read / write request counters")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
