@@ -81,7 +81,6 @@ static_assert(NF == 32, "feature layout assumes two 16-input k-tiles");
 constexpr int kFragW1 = 0;   // 4 fragments: mt
 constexpr int kFragW2 = 4;   // 8 fragments: 2 * mt + s
 constexpr int kFragW3 = 12;  // 2 fragments: s
-constexpr int kFrags = 14;   // bf16x8 fragments of 64 lanes
 constexpr float kLog2e = 1.4426950408889634f;
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
