@@ -77,9 +77,13 @@ def cpu_baseline(pos, m, theta, eps, n_targets, seed=123):
     params = O.make_params(theta=theta, softening=eps)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     ref = O.ref_octree_forces_subset(x, t, params, nthreads=threads)
+    single = None
     if ref is not None:
         _, t_build, t_walk = ref
         kind = "reference"
+        # one thread: the same build and the walk on the first 1,024 targets
+        one = O.ref_octree_forces_subset(x, t[:1024], params, nthreads=1)
+        single = n / (one[1] + one[2] * (n / 1024))
     else:
         t0 = time.perf_counter()
         O.octree_forces_subset(x, t[:1], params, nthreads=threads)
@@ -94,11 +98,13 @@ def cpu_baseline(pos, m, theta, eps, n_targets, seed=123):
         "unit": "particle-steps/s",
         "cores": threads,
         "kind": kind,
+        "single_thread_value": single,
         "cpu_model": cpu_model(),
         "sample": (f"octree_cpu (theta {theta}, eps {eps}, monopole, FP32) on the same state: "
                    f"serial tree build over all {n} particles ({t_build:.2f} s) + walk on "
                    f"{n_targets} random targets ({t_walk:.2f} s, {threads} threads), "
-                   f"extrapolated to one full force pass = one Euler step"),
+                   f"extrapolated to one full force pass = one Euler step; single thread: the "
+                   f"same build and 1,024 of the targets, extrapolated the same way"),
     }
 
 
