@@ -79,3 +79,59 @@ def direct(x, pos, mass, eps):
     d = np.asarray(pos, np.float64)[None, :, :] - x[:, None, :]
     r2 = (d * d).sum(-1) + eps * eps
     return (d * (np.asarray(mass, np.float64)[None, :] / r2 ** 1.5)[:, :, None]).sum(1)
+
+
+# ---- TreePM short-range cells ------------------------------------------------
+# The second-order expansion about a node's COM of the split short-range
+# kernel F(d) = g(R) d, g = w(R) / R^3, R^2 = |d|^2 + eps^2, with the erfc
+# weight w(R) = erfc(R / 2 r_s) + R / (r_s sqrt(pi)) exp(-R^2 / 4 r_s^2)
+# (engine/physics/octree/force/OctreeForce.cpp:49-57). Summed over the node's
+# particles at com + y (sum m y = 0, S = sum m y y^T):
+#   a = M F(d) + 1/2 S_jk d_j d_k F(d),
+# with d_j F_i = G1 d_i d_j + g delta_ij and
+# d_k d_j F_i = G2 d_i d_j d_k + G1 (delta_jk d_i + delta_ik d_j + delta_ij d_k),
+# G1 = g'(R) / R, G2 = G1'(R) / R. The derivatives of g are taken here by
+# central differences of g in R (step 1e-4 R), so this restatement does not
+# share the closed forms it checks.
+
+def tpm_weight(R, rs):
+    from math import erfc, exp, pi, sqrt
+    return erfc(0.5 * R / rs) + R / (rs * sqrt(pi)) * exp(-R * R / (4.0 * rs * rs))
+
+
+def tpm_cell_accel(x, com, mass, quad6, trs, eps, rs):
+    """Second-order expansion of the split kernel of a node (mass, COM,
+    traceless quadrupole quad6 = xx xy xz yy yz zz, trs = sum m |y|^2) at the
+    targets x; d = com - x."""
+    x = np.atleast_2d(np.asarray(x, np.float64))
+    qxx, qxy, qxz, qyy, qyz, qzz = np.asarray(quad6, np.float64)
+    Q = np.array([[qxx, qxy, qxz], [qxy, qyy, qyz], [qxz, qyz, qzz]])
+    S = (Q + trs * I3) / 3.0
+    g = lambda R: tpm_weight(R, rs) / R ** 3
+    out = np.zeros((x.shape[0], 3))
+    for n, xi in enumerate(x):
+        d = np.asarray(com, np.float64) - xi
+        R = float(np.sqrt(d @ d + eps * eps))
+        h = 1e-4 * R
+        g0 = g(R)
+        gp = (g(R + h) - g(R - h)) / (2 * h)
+        G1 = gp / R
+        G1f = lambda r: (g(r + h) - g(r - h)) / (2 * h) / r
+        G2 = (G1f(R + h) - G1f(R - h)) / (2 * h) / R
+        dSd = d @ S @ d
+        trS = np.trace(S)
+        Sd = S @ d
+        out[n] = mass * g0 * d + 0.5 * (G2 * dSd * d + G1 * (trS * d + 2.0 * Sd))
+    return out
+
+
+def tpm_direct(x, pos, mass, eps, rs):
+    """FP64 split-kernel force of particles at x (no cutoff)."""
+    x = np.atleast_2d(np.asarray(x, np.float64))
+    out = np.zeros((x.shape[0], 3))
+    for n, xi in enumerate(x):
+        d = np.asarray(pos, np.float64) - xi
+        R = np.sqrt(np.sum(d * d, axis=1) + eps * eps)
+        w = np.array([tpm_weight(r, rs) for r in R])
+        out[n] = np.sum((np.asarray(mass, np.float64) * w / R ** 3)[:, None] * d, axis=0)
+    return out

@@ -155,3 +155,65 @@ def test_particle_mesh_matches_oracle():
     # the force pass is the short-range walk plus the mesh
     np.testing.assert_allclose(a_tot, a_short + a_pm, rtol=1e-4,
                                atol=1e-5 * np.abs(a_tot).max())
+
+
+@pytest.mark.parametrize("eps", [0.01, 0.3])
+def test_split_kernel_cells_match_expansion_oracle(eps):
+    """Two source clusters, each the only content of one root octant, and 40
+    light targets in a third, all within the cutoff: at theta 0.7 both source
+    octants are accepted whole, so the short-range walk must equal the FP64
+    second-order expansion of the split kernel about each cluster's COM
+    (oracle/expansion.py tpm_cell_accel, which takes the kernel's derivatives
+    numerically) to FP32 rounding. The expansion itself differs from the
+    direct split sum by its third-order error (more than 30x the
+    tolerance at every target), and from the Newtonian expansion times the weight at the COM
+    by more still."""
+    import blitzar_amd as B
+    from oracle import expansion as X
+    from oracle import mlp
+    print(f"case split-kernel cells vs expansion eps={eps}", flush=True)
+    rng = np.random.default_rng(43)
+
+    def box(n, c, h):
+        return np.asarray(c) + rng.uniform(-1.0, 1.0, (n, 3)) * np.asarray(h)
+
+    tgt = box(40, (-1.0, -1.0, -1.0), (0.4, 0.4, 0.4))
+    src_a = box(100, (1.2, 1.2, 1.2), (0.3, 0.1, 0.15))
+    src_b = box(80, (1.2, -1.0, 1.2), (0.1, 0.25, 0.1))
+    m_a = rng.uniform(0.5, 1.5, 100) * 0.01
+    m_b = rng.uniform(0.5, 1.5, 80) * 0.01
+    pos = np.vstack([tgt, src_a, src_b])
+    m = np.concatenate([np.full(40, 1e-15), m_a, m_b]).astype(np.float32)
+    rs = float(np.float32(1.5))
+    with B.OctreeHip(len(pos), theta=0.7, softening=eps, treepm_split_scale=rs,
+                     treepm_cutoff=4.5 * rs) as s:
+        s.set_timing(True)
+        s.upload(pos, None, m)
+        acc = s.compute_accelerations()[:40].astype(np.float64)
+        tm = s.timings()
+    e = float(np.float32(eps))
+    want = np.zeros((40, 3))
+    direct = np.zeros((40, 3))
+    newton_w = np.zeros((40, 3))
+    for xs, ms in ((src_a, m_a), (src_b, m_b)):
+        ms = ms.astype(np.float32).astype(np.float64)
+        M, com, q, trs = mlp.node_moments(xs, ms)
+        want += X.tpm_cell_accel(tgt, com, M, q, trs, e, rs)
+        direct += X.tpm_direct(tgt, xs, ms, e, rs)
+        R = np.sqrt(np.sum((com - tgt) ** 2, axis=1) + e * e)
+        newton_w += mlp.multipole_accel(tgt, com, M, q, e, trs) * np.array(
+            [X.tpm_weight(r, rs) for r in R])[:, None]
+    scale = np.linalg.norm(want, axis=1)
+    d_gpu = np.linalg.norm(acc - want, axis=1) / scale
+    d_trunc = np.linalg.norm(direct - want, axis=1) / scale
+    d_prod = np.linalg.norm(direct - newton_w, axis=1) / scale
+    print(f"  GPU vs expansion max {d_gpu.max():.2e}; expansion vs direct max "
+          f"{d_trunc.max():.2e}; Newton x w vs direct median {np.median(d_prod):.2e}; "
+          f"pc per walk {tm['pc_interactions']}", flush=True)
+    assert tm["pc_interactions"] >= 80  # the clusters entered as cells
+    tol = 2e-6  # FP32 rounding of the walk's cell formula
+    assert d_gpu.max() < tol
+    # every target tells the expansion from the direct split sum, and the
+    # Newtonian expansion weighted at the COM is further off still
+    assert d_trunc.min() > 30 * tol
+    assert np.median(d_prod) > 5 * np.median(d_trunc)

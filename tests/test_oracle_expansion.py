@@ -64,3 +64,46 @@ def test_cluster_field_error_orders():
     assert errs[1] < errs[0] and errs[2] < errs[1]
     rel = errs[0] / np.linalg.norm(ex.direct(x0, pos, m, 0.0)[0])
     assert rel < 3e-3
+
+
+@pytest.mark.parametrize("eps", [0.0, 0.05])
+def test_tpm_cell_expansion_is_the_newton_one_without_a_split(eps):
+    # r_s -> infinity: w = 1, the softened Newtonian second-order expansion
+    pos, m, M, com, q, trs = _cluster(3, size=0.3)
+    x = com + np.array([[1.3, -0.4, 0.8], [-0.9, 1.1, 0.2]])
+    got = ex.tpm_cell_accel(x, com, M, q, trs, eps, 1e7)
+    want = mlp.multipole_accel(x, com, M, q, eps, trs)
+    np.testing.assert_allclose(got, want, rtol=1e-6)
+
+
+@pytest.mark.parametrize("eps", [0.0, 0.05])
+def test_tpm_cell_expansion_error_is_third_order(eps):
+    # a cluster of size a at 1.5 r_s from the target: the error against the
+    # direct split sum falls by ~8 per halving of a (second-order expansion),
+    # and a single particle is exact
+    rs = 1.0
+    dirn = np.array([0.48, 0.6, 0.64])
+    errs = []
+    for size in (0.4, 0.2, 0.1):
+        pos, m, M, com, q, trs = _cluster(4, n=25, size=size)
+        x = com - 1.5 * rs * dirn
+        exact = ex.tpm_direct(x, pos, m, eps, rs)
+        got = ex.tpm_cell_accel(x, com, M, q, trs, eps, rs)
+        errs.append(np.linalg.norm(got - exact) / np.linalg.norm(exact))
+    assert 6.0 < errs[0] / errs[1] < 10.0 and 6.0 < errs[1] / errs[2] < 10.0
+    one = ex.tpm_cell_accel(np.zeros((1, 3)), dirn, 1.3, np.zeros(6), 0.0, eps, rs)
+    np.testing.assert_allclose(one, ex.tpm_direct(np.zeros((1, 3)), dirn[None, :], [1.3], eps, rs),
+                               rtol=1e-9)
+    # multiplying the Newtonian expansion by w(R) instead is only second
+    # order: its error falls by ~4.5 per halving where the expansion's falls
+    # by ~8
+    prod_err, cell_err = [], []
+    for size in (0.1, 0.05):
+        pos, m, M, com, q, trs = _cluster(4, n=25, size=size)
+        x = com - 1.5 * rs * dirn
+        exact = ex.tpm_direct(x, pos, m, eps, rs)
+        prod = mlp.multipole_accel(x, com, M, q, eps, trs) * ex.tpm_weight(
+            float(np.sqrt(1.5 ** 2 + eps * eps)), rs)
+        prod_err.append(np.linalg.norm(prod - exact))
+        cell_err.append(np.linalg.norm(ex.tpm_cell_accel(x, com, M, q, trs, eps, rs) - exact))
+    assert prod_err[0] / prod_err[1] < 5.5 < 6.5 < cell_err[0] / cell_err[1]
