@@ -218,6 +218,7 @@ bool alloc_nodes(bh_hip_handle* h, size_t cap)
     dfree(h, d.scan_tmp);
     dfree(h, d.bnd_tmp);
     dfree(h, d.big_list);
+    dfree(h, d.topo_cnt);
     dfree(h, d.scan_blocks);
     dfree(h, d.nparent);
     dfree(h, d.gscan);
@@ -227,6 +228,7 @@ bool alloc_nodes(bh_hip_handle* h, size_t cap)
     d.noct = nullptr;
     d.nprefix = nullptr;
     d.nstart = d.ncount = d.nchild = d.nfirst = d.scan_tmp = d.bnd_tmp = d.big_list = nullptr;
+    d.topo_cnt = nullptr;
     d.scan_blocks = d.nparent = d.gscan = d.gscan_blocks = nullptr;
     d.nlevel = nullptr;
     d.wnode = nullptr;
@@ -240,6 +242,7 @@ bool alloc_nodes(bh_hip_handle* h, size_t cap)
            dalloc(h, &d.scan_tmp, scan_n, "scan tmp") &&
            dalloc(h, &d.bnd_tmp, 8 * scan_n, "child bounds") &&
            dalloc(h, &d.big_list, scan_n, "large-node list") &&
+           dalloc(h, &d.topo_cnt, bh::topo_count_words(d.n), "tile node counts") &&
            dalloc_zero(h, &d.scan_blocks, bh::scan_block_sums_words(scan_n), "scan blocks",
                        d.stream) &&
            dalloc(h, &d.nparent, cap, "node parent") && dalloc(h, &d.gscan, cap, "group scan") &&
@@ -297,7 +300,8 @@ bool zero_err(bh_hip_handle* h)
 
 // Device errors that no retry repairs (a node-capacity overflow is repaired
 // inside the force evaluation, see check_build).
-constexpr int kFatalBits = bh::kErrStackOverflow | bh::kErrLeafTooLarge | bh::kErrScan;
+constexpr int kFatalBits =
+    bh::kErrStackOverflow | bh::kErrLeafTooLarge | bh::kErrScan | bh::kErrTopology;
 
 // Synchronise and turn device error bits into a message.
 bh_hip_result_t finish(bh_hip_handle* h, const char* what)
@@ -324,6 +328,8 @@ bh_hip_result_t finish(bh_hip_handle* h, const char* what)
             m += " leaf larger than 2^23 particles;";
         if (flags & bh::kErrScan)
             m += " device scan did not complete (results of this call invalid);";
+        if (flags & bh::kErrTopology)
+            m += " tree links inconsistent (results of this call invalid);";
         if (h->steps_done >= 0)
             m += " stopped after " + std::to_string(h->steps_done) +
                  " completed steps (the state holds them and part of the next)";

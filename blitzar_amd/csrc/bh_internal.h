@@ -107,7 +107,16 @@ enum DeviceError : int {
     kErrLeafTooLarge = 4,
     kErrKeyMode = 8,
     kErrScan = 16,  // a scan block gave up waiting for its predecessors
+    kErrTopology = 32,  // a tree link the build could not resolve
 };
+
+// One-pass tree topology (k_tree.hip): tiles of kTopoTile sorted keys; it
+// handles buckets of up to kTopoHalo particles (wider ones build level by
+// level). Its per-tile, per-level node counts need topo_count_words(n) words.
+constexpr int kTopoTile = 4096;
+constexpr int kTopoHalo = 64;
+inline uint32_t topo_tiles(size_t n) { return static_cast<uint32_t>((n + kTopoTile - 1) / kTopoTile); }
+inline size_t topo_count_words(size_t n) { return static_cast<size_t>(kLevels) * (topo_tiles(n) + 1); }
 
 // Parameters the walk needs (by value into the kernel).
 struct WalkParams {
@@ -201,6 +210,7 @@ struct Device {
     uint32_t* scan_tmp = nullptr;  // per-level scan scratch
     uint32_t* bnd_tmp = nullptr;   // per-level child bounds (8 per node) from count to emit
     uint32_t* big_list = nullptr;  // per-level indices of nodes searched by a whole wavefront
+    uint32_t* topo_cnt = nullptr;  // one-pass topology: node counts per tile and level
     uint32_t* scan_blocks = nullptr;
     int* level_info = nullptr;  // [0..22) base, [22..44) count, [44] total, [45] max level, [46] groups
 

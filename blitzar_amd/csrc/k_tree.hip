@@ -341,6 +341,345 @@ __global__ __launch_bounds__(256) void tree_emit_kernel(
     }
 }
 
+// One-pass topology. Every node of the tree is a run of equal level-L key
+// prefixes, so it is named by its level L and its first particle s. With d_p
+// the number of leading 3-bit digits that sorted keys p - 1 and p share
+// (0..21), particle p starts a run at level L exactly when d_p < L (particle
+// 0 starts every run). A run at level l holding more than B particles
+// contains B + 1 consecutive particles whose B inner boundaries all share at
+// least l digits. So with
+//     w_j = min(d_{j+1}, ..., d_{j+B})   over windows inside [0, n)
+//     W_i = max(w_j : i - B <= j <= i)   (-1 when no window holds i)
+// the run at level l that holds particle i has more than B particles exactly
+// when W_i >= l. A node exists when every ancestor split; a node at level L
+// splits when L < 21 and (L < D or it holds more than B particles). Hence the
+// nodes that start at particle s are those of levels
+//     lo_s = d_s + 1 (0 for s = 0)  ...  hi_s = min(21, max(D, W_s + 1)),
+// the one at hi_s a leaf and each other the parent of the next. Level L's
+// nodes in breadth-first order are the particles with lo_s <= L <= hi_s in
+// increasing order, so a node's index is its level's base plus its rank
+// among them. Passes: per 4,096-particle tile, count nodes per level; scan
+// the counts per level; per tile, emit the nodes; then, per node, link what
+// a tile cannot see (parents of later siblings, child counts, counts of
+// internal nodes and of leaves longer than B).
+constexpr int kTopoThreads = 256;
+constexpr int kTopoPer = kTopoTile / kTopoThreads;  // consecutive particles per thread
+constexpr uint32_t kPending = 0xfffffffeu;
+constexpr unsigned long long kKeyMask = 0x7fffffffffffffffull;
+
+__device__ inline int shared_digits(unsigned long long a, unsigned long long b)
+{
+    const unsigned long long x = (a ^ b) & kKeyMask;
+    return x == 0ull ? kMaxLevel : (__clzll(x) - 1) / 3;
+}
+
+struct TopoTile {
+    int8_t d[kTopoTile + 2 * kTopoHalo];  // d at particle t0 - H + m (-1 outside [1, n))
+    int8_t w[kTopoTile + kTopoHalo];      // w at window start t0 - H + m
+    int8_t lo[kTopoTile];
+    int8_t hi[kTopoTile];
+    uint32_t cnt[kLevels][kTopoThreads];  // nodes per level and thread
+};
+
+// d, w, the level range [lo, hi] of every particle of the tile starting at t0
+// (levels above lmax dropped) and each thread's node count per level.
+__device__ inline void topo_tile(TopoTile& S, const unsigned long long* __restrict__ keys,
+                                 uint32_t n, uint32_t t0, int B, int D, int lmax)
+{
+    const int tid = threadIdx.x;
+    for (int m = tid; m < kTopoTile + 2 * kTopoHalo; m += kTopoThreads) {
+        const long long p = static_cast<long long>(t0) - kTopoHalo + m;
+        int v = -1;
+        if (p >= 1 && p < static_cast<long long>(n))
+            v = shared_digits(keys[p - 1], keys[p]);
+        S.d[m] = static_cast<int8_t>(v);
+    }
+    for (int L = 0; L < kLevels; ++L)
+        S.cnt[L][tid] = 0u;
+    __syncthreads();
+    for (int m = tid; m < kTopoTile + kTopoHalo; m += kTopoThreads) {
+        int v = kMaxLevel;
+        for (int b = 1; b <= B; ++b)  // m + B < kTopoTile + 2 * kTopoHalo
+            v = min(v, static_cast<int>(S.d[m + b]));
+        S.w[m] = static_cast<int8_t>(v);
+    }
+    __syncthreads();
+    for (int q = 0; q < kTopoPer; ++q) {
+        const int r = tid * kTopoPer + q;
+        const uint32_t i = t0 + static_cast<uint32_t>(r);
+        int lo = kLevels, hi = -1;
+        if (i < n) {
+            int W = -1;
+            for (int b = 0; b <= B; ++b)  // r + H - b >= 0
+                W = max(W, static_cast<int>(S.w[r + kTopoHalo - b]));
+            lo = i == 0 ? 0 : static_cast<int>(S.d[r + kTopoHalo]) + 1;
+            hi = min(min(kMaxLevel, max(D, W + 1)), lmax);
+        }
+        S.lo[r] = static_cast<int8_t>(lo);
+        S.hi[r] = static_cast<int8_t>(hi);
+        for (int L = lo; L <= hi; ++L)
+            S.cnt[L][tid] += 1u;
+    }
+    __syncthreads();
+}
+
+__device__ inline uint32_t wave_total(uint32_t v)
+{
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Nodes per level in each tile: tilecnt[L * ntiles + tile].
+__global__ __launch_bounds__(kTopoThreads) void topo_count_kernel(
+    const unsigned long long* __restrict__ keys, uint32_t n, int B, int D,
+    uint32_t* __restrict__ tilecnt, uint32_t ntiles)
+{
+    __shared__ TopoTile S;
+    const uint32_t tile = blockIdx.x;
+    topo_tile(S, keys, n, tile * kTopoTile, B, D, kMaxLevel);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int L = wv; L < kLevels; L += kTopoThreads / 64) {
+        const uint32_t v = wave_total(S.cnt[L][lane] + S.cnt[L][lane + 64] +
+                                      S.cnt[L][lane + 128] + S.cnt[L][lane + 192]);
+        if (lane == 0)
+            tilecnt[static_cast<size_t>(L) * ntiles + tile] = v;
+    }
+}
+
+// Exclusive scan of each level's tile counts (one block per level, in place);
+// the level totals go to tilecnt[kLevels * ntiles + L].
+__global__ __launch_bounds__(kScanBlock) void topo_scan_kernel(uint32_t* __restrict__ tilecnt,
+                                                               uint32_t ntiles)
+{
+    __shared__ uint32_t tmp[kScanWaves];
+    const int L = blockIdx.x;
+    uint32_t* col = tilecnt + static_cast<size_t>(L) * ntiles;
+    uint32_t carry = 0;
+    for (uint32_t b = 0; b < ntiles; b += kScanBlock) {
+        const uint32_t t = b + threadIdx.x;
+        const uint32_t v = t < ntiles ? col[t] : 0u;
+        uint32_t total = 0;
+        const uint32_t ex = block_exclusive256(v, tmp, &total);
+        if (t < ntiles)
+            col[t] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0)
+        tilecnt[static_cast<size_t>(kLevels) * ntiles + L] = carry;
+}
+
+// The nodes that start in one tile, in breadth-first place. The first level
+// whose nodes would not fit in node_cap is dropped with every deeper one (the
+// nodes above it that would split stay leaves; flagged). Block 0 also writes
+// the level table.
+__global__ __launch_bounds__(kTopoThreads) void topo_emit_kernel(
+    const unsigned long long* __restrict__ keys, uint32_t n, int B, int D,
+    const uint32_t* __restrict__ tilecnt, uint32_t ntiles, int node_cap,
+    uint64_t* __restrict__ nprefix, uint8_t* __restrict__ nlevel, uint32_t* __restrict__ nstart,
+    uint32_t* __restrict__ ncount, uint32_t* __restrict__ nchild, uint32_t* __restrict__ nfirst,
+    uint32_t* __restrict__ nparent, int* __restrict__ info, int* __restrict__ err)
+{
+    __shared__ TopoTile S;
+    __shared__ uint32_t sbase[kLevels];  // level base + this tile's offset
+    __shared__ int slmax;
+    const int tid = threadIdx.x;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t t0 = tile * kTopoTile;
+    const uint32_t* tot = tilecnt + static_cast<size_t>(kLevels) * ntiles;
+    if (tid == 0)
+        slmax = kMaxLevel;
+    __syncthreads();
+    uint64_t below = 0;  // nodes of the levels above level tid
+    if (tid < kLevels) {
+        for (int l = 0; l < tid; ++l)
+            below += tot[l];
+        if (below + tot[tid] > static_cast<uint64_t>(node_cap))
+            atomicMin(&slmax, tid - 1);  // level 0 (the root) always fits
+    }
+    __syncthreads();
+    const int lmax = slmax;
+    if (tid < kLevels) {
+        const uint32_t base = static_cast<uint32_t>(below);
+        sbase[tid] = base + (tid <= lmax ? tilecnt[static_cast<size_t>(tid) * ntiles + tile] : 0u);
+        if (tile == 0) {
+            const bool kept = tid <= lmax;
+            uint64_t kept_total = 0;
+            for (int l = 0; l <= lmax; ++l)
+                kept_total += tot[l];
+            info[tid] = kept ? static_cast<int>(base) : static_cast<int>(kept_total);
+            info[kLevels + tid] = kept ? static_cast<int>(tot[tid]) : 0;
+            if (tid == 0) {
+                int deepest = 0;
+                for (int l = 0; l <= lmax; ++l)
+                    if (tot[l] > 0u)
+                        deepest = l;
+                info[kInfoTotal] = static_cast<int>(kept_total);
+                info[kInfoMaxLevel] = deepest;
+                info[kInfoBig] = 0;
+                if (lmax < kMaxLevel && tot[lmax + 1] > 0u)
+                    atomicOr(err, kErrNodeOverflow);
+            }
+        }
+    }
+    topo_tile(S, keys, n, t0, B, D, lmax);  // begins and ends with a barrier-separated phase
+    // per level: exclusive scan of the thread counts, in place
+    const int lane = tid & 63, wv = tid >> 6;
+    for (int L = wv; L < kLevels; L += kTopoThreads / 64) {
+        const uint32_t a0 = S.cnt[L][4 * lane], a1 = S.cnt[L][4 * lane + 1];
+        const uint32_t a2 = S.cnt[L][4 * lane + 2], a3 = S.cnt[L][4 * lane + 3];
+        const uint32_t sum = a0 + a1 + a2 + a3;
+        const uint32_t ex = wave_inclusive(sum) - sum;
+        S.cnt[L][4 * lane] = ex;
+        S.cnt[L][4 * lane + 1] = ex + a0;
+        S.cnt[L][4 * lane + 2] = ex + a0 + a1;
+        S.cnt[L][4 * lane + 3] = ex + a0 + a1 + a2;
+    }
+    __syncthreads();
+    for (int q = 0; q < kTopoPer; ++q) {
+        const int r = tid * kTopoPer + q;
+        const int lo = S.lo[r], hi = S.hi[r];
+        if (lo > hi)
+            continue;
+        const uint32_t i = t0 + static_cast<uint32_t>(r);
+        const unsigned long long key = keys[i] & kKeyMask;
+        uint32_t prev = 0xffffffffu;
+        for (int L = lo; L <= hi; ++L) {
+            const uint32_t idx = sbase[L] + S.cnt[L][tid];
+            S.cnt[L][tid] += 1u;
+            nprefix[idx] = L == 0 ? 0ull : key >> (3 * (kMaxLevel - L));
+            nlevel[idx] = static_cast<uint8_t>(L);
+            nstart[idx] = i;
+            if (L < hi) {
+                // the first child starts here too: its rank at L + 1
+                nfirst[idx] = sbase[L + 1] + S.cnt[L + 1][tid];
+                nchild[idx] = kPending;
+                ncount[idx] = kPending;
+            } else {
+                nfirst[idx] = i;
+                nchild[idx] = 0u;
+                // a leaf below level 21 that was not cut by lmax holds at most
+                // B particles: its end is the next run start within B
+                uint32_t c = kPending;
+                for (int b = 1; b <= B; ++b) {
+                    if (S.d[r + kTopoHalo + b] < L) {  // -1 at and beyond n
+                        c = static_cast<uint32_t>(b);
+                        break;
+                    }
+                }
+                ncount[idx] = c;
+            }
+            nparent[idx] = L == 0 ? 0xffffffffu : (L > lo ? prev : kPending);
+            prev = idx;
+        }
+    }
+}
+
+// First particle past s whose level-L prefix differs from s's (or n):
+// galloping, then bisection.
+__device__ inline uint32_t run_end(const unsigned long long* __restrict__ keys, uint32_t n,
+                                   uint32_t s, int L)
+{
+    const int sh = 3 * (kMaxLevel - L);
+    const unsigned long long k0 = (keys[s] & kKeyMask) >> sh;
+    uint32_t in = s, out = n;
+    for (uint64_t step = 1;; step <<= 1) {
+        const uint64_t p = static_cast<uint64_t>(in) + step;
+        if (p >= n)
+            break;
+        if (((keys[p] & kKeyMask) >> sh) != k0) {
+            out = static_cast<uint32_t>(p);
+            break;
+        }
+        in = static_cast<uint32_t>(p);
+    }
+    while (out - in > 1u) {
+        const uint32_t mid = in + (out - in) / 2u;
+        if (((keys[mid] & kKeyMask) >> sh) != k0)
+            out = mid;
+        else
+            in = mid;
+    }
+    return out;
+}
+
+// Number of children of an internal node (level L, prefix p) whose first
+// child is f: the run of level-(L + 1) nodes from f with parent prefix p.
+__device__ inline uint32_t topo_children(const uint64_t* __restrict__ nprefix,
+                                         const int* __restrict__ info, int L,
+                                         unsigned long long p, uint32_t f)
+{
+    const uint32_t end =
+        static_cast<uint32_t>(info[L + 1]) + static_cast<uint32_t>(info[kLevels + L + 1]);
+    uint32_t m = 1;
+    while (m < 8u && f + m < end && (nprefix[f + m] >> 3) == p)
+        ++m;
+    return m;
+}
+
+// What the tiles left pending, one thread per node. Only pending fields are
+// written; every field this pass reads is final or is read only to tell a
+// leaf (nchild 0) from an internal node (pending or final, never 0).
+__global__ __launch_bounds__(256) void topo_link_kernel(
+    const unsigned long long* __restrict__ keys, uint32_t n, const uint64_t* __restrict__ nprefix,
+    const uint8_t* __restrict__ nlevel, const uint32_t* __restrict__ nstart,
+    uint32_t* __restrict__ ncount, uint32_t* __restrict__ nchild,
+    const uint32_t* __restrict__ nfirst, uint32_t* __restrict__ nparent,
+    const int* __restrict__ info, int* __restrict__ err)
+{
+    const int total = info[kInfoTotal];
+    for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < total; x += gridDim.x * blockDim.x) {
+        const int L = nlevel[x];
+        const unsigned long long px = nprefix[x];
+        if (nparent[x] == kPending) {
+            // a later sibling: its parent is that of the first sibling (the
+            // parent's first child, linked by the emit pass), at most 7 back
+            const int lb = info[L];
+            int y = x;
+            for (int k = 0; k < 7 && y - 1 >= lb && (nprefix[y - 1] >> 3) == (px >> 3); ++k)
+                --y;
+            const uint32_t par = nparent[y];
+            if (y == x || par == kPending)
+                atomicOr(err, kErrTopology);
+            else
+                nparent[x] = par;
+        }
+        if (nchild[x] == kPending) {
+            const uint32_t f = nfirst[x];
+            if (L >= kMaxLevel || f >= static_cast<uint32_t>(total)) {
+                atomicOr(err, kErrTopology);
+                continue;
+            }
+            nchild[x] = topo_children(nprefix, info, L, px, f);
+        }
+        if (ncount[x] == kPending) {
+            // the run ends where its last leaf ends: descend by last children
+            uint32_t cur = static_cast<uint32_t>(x);
+            int lc = L;
+            unsigned long long pc = px;
+            bool ok = true;
+            for (int g = 0; g < kLevels && nchild[cur] != 0u; ++g) {
+                const uint32_t f = nfirst[cur];
+                if (lc >= kMaxLevel || f >= static_cast<uint32_t>(total)) {
+                    ok = false;
+                    break;
+                }
+                cur = f + topo_children(nprefix, info, lc, pc, f) - 1u;
+                ++lc;
+                pc = nprefix[cur];
+            }
+            if (!ok || nchild[cur] != 0u) {
+                atomicOr(err, kErrTopology);
+                continue;
+            }
+            const uint32_t cs = nstart[cur];
+            const uint32_t cc = ncount[cur];
+            const uint32_t e = cc != kPending ? cs + cc : run_end(keys, n, cs, lc);
+            ncount[x] = e - nstart[x];
+        }
+    }
+}
+
 // Third moments about the COM, T_ijk = sum m y_i y_j y_k (10 components:
 // xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz), accumulated in registers.
 struct Oct {
@@ -394,28 +733,31 @@ __device__ inline void oct_add_child(Oct& o, const float* tc, const WalkNode& c,
     }
 }
 
-// Moments of one level (deepest level first). Leaves also pack their
-// particles: spart[s] = (FP32 offset from the leaf lattice centre, mass) and
-// sleaf[s] = leaf node. OCT: also the third moments (noct, 10 per node; read
-// by the learned far-field correction).
-template <bool OCT>
-__global__ __launch_bounds__(256) void tree_moments_kernel(
-    const uint64_t* __restrict__ nprefix, const uint32_t* __restrict__ nstart,
-    const uint32_t* __restrict__ ncount, const uint32_t* __restrict__ nchild,
-    const uint32_t* __restrict__ nfirst, const int* __restrict__ info, int level,
+// Moments of one node. Leaves also pack their particles: spart[s] = (FP32
+// offset from the leaf lattice centre, mass) and sleaf[s] = leaf node. An
+// internal node combines its children, which must be complete. OCT: also the
+// third moments (noct, 10 per node; read by the learned far-field correction).
+// |d|^2 with explicit roundings: the moments come out the same bits in every
+// kernel that computes them, whatever the compiler fuses around them.
+__device__ inline float norm2_rn(float x, float y, float z)
+{
+    return __fmaf_rn(z, z, __fmaf_rn(y, y, __fmul_rn(x, x)));
+}
+
+template <bool OCT, bool LEAF>
+__device__ inline void node_moments(
+    int node, int level, const uint64_t* __restrict__ nprefix, const uint32_t* __restrict__ ncount,
+    const uint32_t* __restrict__ nchild, const uint32_t* __restrict__ nfirst,
     const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
     const float* __restrict__ pm, const RootGeom* __restrict__ geom, WalkNode* __restrict__ wn,
     float4* __restrict__ spart, uint32_t* __restrict__ sleaf, float* __restrict__ ext,
     float* __restrict__ noct, int refit, int* err)
 {
-    const int base = info[level];
-    const int cnt = info[kLevels + level];
     const float ux = geom->unit_f[0], uy = geom->unit_f[1], uz = geom->unit_f[2];
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
-        const int node = base + k;
+    {
         int c[3];
         lattice_centre(nprefix[node], level, c);
-        const uint32_t nc = nchild[node];
+        const uint32_t nc = LEAF ? 0u : nchild[node];
         const uint32_t first = nfirst[node];
         float M = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
         float qxx = 0.0f, qxy = 0.0f, qxz = 0.0f, qyy = 0.0f, qyz = 0.0f, bmax = 0.0f, trs = 0.0f;
@@ -427,11 +769,11 @@ __global__ __launch_bounds__(256) void tree_moments_kernel(
         Oct oc;
         if (OCT)
             oct_zero(oc);
-        if (nc == 0) {
+        if (LEAF) {
             count = ncount[node];
-            const double ox = geom->lo_d[0] + static_cast<double>(c[0]) * geom->unit_d[0];
-            const double oy = geom->lo_d[1] + static_cast<double>(c[1]) * geom->unit_d[1];
-            const double oz = geom->lo_d[2] + static_cast<double>(c[2]) * geom->unit_d[2];
+            const double ox = __fma_rn(static_cast<double>(c[0]), geom->unit_d[0], geom->lo_d[0]);
+            const double oy = __fma_rn(static_cast<double>(c[1]), geom->unit_d[1], geom->lo_d[1]);
+            const double oz = __fma_rn(static_cast<double>(c[2]), geom->unit_d[2], geom->lo_d[2]);
             float sx = 0.0f, sy = 0.0f, sz = 0.0f;
 #pragma unroll 4
             for (uint32_t s = first; s < first + count; ++s) {
@@ -442,10 +784,10 @@ __global__ __launch_bounds__(256) void tree_moments_kernel(
                 spart[s] = make_float4(x, y, z, m);
                 sleaf[s] = static_cast<uint32_t>(node);
                 e = fmaxf(e, fmaxf(fabsf(x) / ux, fmaxf(fabsf(y) / uy, fabsf(z) / uz)));
-                M += m;
-                sx += m * x;
-                sy += m * y;
-                sz += m * z;
+                M = __fadd_rn(M, m);
+                sx = __fmaf_rn(m, x, sx);
+                sy = __fmaf_rn(m, y, sy);
+                sz = __fmaf_rn(m, z, sz);
             }
             if (M > 0.0f) {
                 cx = sx / M;
@@ -455,14 +797,16 @@ __global__ __launch_bounds__(256) void tree_moments_kernel(
 #pragma unroll 4
             for (uint32_t s = first; s < first + count; ++s) {
                 const float4 p = spart[s];
-                const float dx = p.x - cx, dy = p.y - cy, dz = p.z - cz;
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                qxx += p.w * (3.0f * dx * dx - r2);
-                qxy += p.w * (3.0f * dx * dy);
-                qxz += p.w * (3.0f * dx * dz);
-                qyy += p.w * (3.0f * dy * dy - r2);
-                qyz += p.w * (3.0f * dy * dz);
-                trs += p.w * r2;
+                const float dx = __fsub_rn(p.x, cx), dy = __fsub_rn(p.y, cy),
+                            dz = __fsub_rn(p.z, cz);
+                const float r2 = norm2_rn(dx, dy, dz);
+                const float tx = __fmul_rn(3.0f, dx), ty = __fmul_rn(3.0f, dy);
+                qxx = __fmaf_rn(p.w, __fmaf_rn(tx, dx, -r2), qxx);
+                qxy = __fmaf_rn(p.w, __fmul_rn(tx, dy), qxy);
+                qxz = __fmaf_rn(p.w, __fmul_rn(tx, dz), qxz);
+                qyy = __fmaf_rn(p.w, __fmaf_rn(ty, dy, -r2), qyy);
+                qyz = __fmaf_rn(p.w, __fmul_rn(ty, dz), qyz);
+                trs = __fmaf_rn(p.w, r2, trs);
                 bmax = fmaxf(bmax, r2);
                 if (OCT)
                     oct_add_point(oc, p.w, dx, dy, dz);
@@ -477,13 +821,13 @@ __global__ __launch_bounds__(256) void tree_moments_kernel(
             for (uint32_t jj = 0; jj < 8; ++jj) {
                 const WalkNode ch = wn[first + (jj < nc ? jj : nc - 1u)];
                 const float cm = jj < nc ? ch.mass : 0.0f;
-                const float rx = static_cast<float>(ch.cx - c[0]) * ux + ch.comx;
-                const float ry = static_cast<float>(ch.cy - c[1]) * uy + ch.comy;
-                const float rz = static_cast<float>(ch.cz - c[2]) * uz + ch.comz;
-                M += cm;
-                sx += cm * rx;
-                sy += cm * ry;
-                sz += cm * rz;
+                const float rx = __fmaf_rn(static_cast<float>(ch.cx - c[0]), ux, ch.comx);
+                const float ry = __fmaf_rn(static_cast<float>(ch.cy - c[1]), uy, ch.comy);
+                const float rz = __fmaf_rn(static_cast<float>(ch.cz - c[2]), uz, ch.comz);
+                M = __fadd_rn(M, cm);
+                sx = __fmaf_rn(cm, rx, sx);
+                sy = __fmaf_rn(cm, ry, sy);
+                sz = __fmaf_rn(cm, rz, sz);
             }
             if (M > 0.0f) {
                 cx = sx / M;
@@ -494,18 +838,23 @@ __global__ __launch_bounds__(256) void tree_moments_kernel(
             for (uint32_t jj = 0; jj < 8; ++jj) {
                 const bool live = jj < nc;
                 const WalkNode ch = wn[first + (live ? jj : nc - 1u)];
-                const float dx = static_cast<float>(ch.cx - c[0]) * ux + ch.comx - cx;
-                const float dy = static_cast<float>(ch.cy - c[1]) * uy + ch.comy - cy;
-                const float dz = static_cast<float>(ch.cz - c[2]) * uz + ch.comz - cz;
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                qxx += live ? ch.qxx + ch.mass * (3.0f * dx * dx - r2) : 0.0f;
-                qxy += live ? ch.qxy + ch.mass * (3.0f * dx * dy) : 0.0f;
-                qxz += live ? ch.qxz + ch.mass * (3.0f * dx * dz) : 0.0f;
-                qyy += live ? ch.qyy + ch.mass * (3.0f * dy * dy - r2) : 0.0f;
-                qyz += live ? ch.qyz + ch.mass * (3.0f * dy * dz) : 0.0f;
-                trs += live ? ch.trs + ch.mass * r2 : 0.0f;
-                if (live && ch.mass > 0.0f)
-                    bmax = fmaxf(bmax, sqrtf(r2) + ch.bmax);
+                const float dx =
+                    __fsub_rn(__fmaf_rn(static_cast<float>(ch.cx - c[0]), ux, ch.comx), cx);
+                const float dy =
+                    __fsub_rn(__fmaf_rn(static_cast<float>(ch.cy - c[1]), uy, ch.comy), cy);
+                const float dz =
+                    __fsub_rn(__fmaf_rn(static_cast<float>(ch.cz - c[2]), uz, ch.comz), cz);
+                const float r2 = norm2_rn(dx, dy, dz);
+                const float tx = __fmul_rn(3.0f, dx), ty = __fmul_rn(3.0f, dy);
+                const float m = ch.mass;
+                qxx = __fadd_rn(qxx, live ? __fmaf_rn(m, __fmaf_rn(tx, dx, -r2), ch.qxx) : 0.0f);
+                qxy = __fadd_rn(qxy, live ? __fmaf_rn(m, __fmul_rn(tx, dy), ch.qxy) : 0.0f);
+                qxz = __fadd_rn(qxz, live ? __fmaf_rn(m, __fmul_rn(tx, dz), ch.qxz) : 0.0f);
+                qyy = __fadd_rn(qyy, live ? __fmaf_rn(m, __fmaf_rn(ty, dy, -r2), ch.qyy) : 0.0f);
+                qyz = __fadd_rn(qyz, live ? __fmaf_rn(m, __fmul_rn(ty, dz), ch.qyz) : 0.0f);
+                trs = __fadd_rn(trs, live ? __fmaf_rn(m, r2, ch.trs) : 0.0f);
+                if (live && m > 0.0f)
+                    bmax = fmaxf(bmax, __fadd_rn(sqrtf(r2), ch.bmax));
                 if (live) {
                     const int off = max(abs(ch.cx - c[0]), max(abs(ch.cy - c[1]), abs(ch.cz - c[2])));
                     e = fmaxf(e, ext[first + jj] + static_cast<float>(off));
@@ -565,6 +914,49 @@ __global__ __launch_bounds__(256) void tree_moments_kernel(
         }
         w.meta = make_meta(nc, static_cast<uint32_t>(wlevel), count);
         wn[node] = w;
+    }
+}
+
+// Leaves of every level in one launch: a leaf depends only on its particles.
+template <bool OCT>
+__global__ __launch_bounds__(256) void leaf_moments_kernel(
+    const uint64_t* __restrict__ nprefix, const uint8_t* __restrict__ nlevel,
+    const uint32_t* __restrict__ ncount, const uint32_t* __restrict__ nchild,
+    const uint32_t* __restrict__ nfirst, const int* __restrict__ info,
+    const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
+    const float* __restrict__ pm, const RootGeom* __restrict__ geom, WalkNode* __restrict__ wn,
+    float4* __restrict__ spart, uint32_t* __restrict__ sleaf, float* __restrict__ ext,
+    float* __restrict__ noct, int refit, int* err)
+{
+    const int total = info[kInfoTotal];
+    for (int node = blockIdx.x * blockDim.x + threadIdx.x; node < total;
+         node += gridDim.x * blockDim.x) {
+        if (nchild[node] != 0)
+            continue;
+        node_moments<OCT, true>(node, nlevel[node], nprefix, ncount, nchild, nfirst, px, py, pz,
+                                pm, geom, wn, spart, sleaf, ext, noct, refit, err);
+    }
+}
+
+// Internal nodes of one level (deepest level first, after the leaves).
+template <bool OCT>
+__global__ __launch_bounds__(256) void tree_moments_kernel(
+    const uint64_t* __restrict__ nprefix, const uint32_t* __restrict__ ncount,
+    const uint32_t* __restrict__ nchild, const uint32_t* __restrict__ nfirst,
+    const int* __restrict__ info, int level, const double* __restrict__ px,
+    const double* __restrict__ py, const double* __restrict__ pz, const float* __restrict__ pm,
+    const RootGeom* __restrict__ geom, WalkNode* __restrict__ wn, float4* __restrict__ spart,
+    uint32_t* __restrict__ sleaf, float* __restrict__ ext, float* __restrict__ noct, int refit,
+    int* err)
+{
+    const int base = info[level];
+    const int cnt = info[kLevels + level];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
+        const int node = base + k;
+        if (nchild[node] == 0)
+            continue;
+        node_moments<OCT, false>(node, level, nprefix, ncount, nchild, nfirst, px, py, pz, pm,
+                                 geom, wn, spart, sleaf, ext, noct, refit, err);
     }
 }
 
@@ -698,6 +1090,22 @@ inline size_t level_bound(int level, size_t n, size_t cap)
 void launch_tree_build(Device& d, int leaf_depth, int bucket)
 {
     const uint32_t n = static_cast<uint32_t>(d.n);
+    const unsigned long long* sorted = reinterpret_cast<const unsigned long long*>(d.keys);
+    if (n > 0 && bucket <= kTopoHalo && d.topo_cnt) {
+        const uint32_t ntiles = topo_tiles(d.n);
+        topo_count_kernel<<<ntiles, kTopoThreads, 0, d.stream>>>(sorted, n, bucket, leaf_depth,
+                                                                 d.topo_cnt, ntiles);
+        topo_scan_kernel<<<kLevels, kScanBlock, 0, d.stream>>>(d.topo_cnt, ntiles);
+        topo_emit_kernel<<<ntiles, kTopoThreads, 0, d.stream>>>(
+            sorted, n, bucket, leaf_depth, d.topo_cnt, ntiles, static_cast<int>(d.node_cap),
+            d.nprefix, d.nlevel, d.nstart, d.ncount, d.nchild, d.nfirst, d.nparent, d.level_info,
+            d.err);
+        topo_link_kernel<<<level_grid(std::min<size_t>(d.node_cap, 2 * d.n + 64)), 256, 0,
+                           d.stream>>>(sorted, n, d.nprefix, d.nlevel, d.nstart, d.ncount,
+                                       d.nchild, d.nfirst, d.nparent, d.level_info, d.err);
+        return;
+    }
+    // Buckets wider than the tiles' halo: level by level, top down.
     tree_root_kernel<<<1, 64, 0, d.stream>>>(d.nprefix, d.nlevel, d.nstart, d.ncount, d.nchild,
                                              d.nfirst, d.nparent, d.level_info, n);
     const unsigned long long* keys = reinterpret_cast<const unsigned long long*>(d.keys);
@@ -735,18 +1143,27 @@ void launch_groups(Device& d)
 
 void launch_tree_moments(Device& d, bool refit)
 {
-    for (int L = kMaxLevel; L >= 0; --L) {
+    const int rf = refit ? 1 : 0;
+    const unsigned gl = level_grid(std::min<size_t>(d.node_cap, d.n));
+    if (d.noct)
+        leaf_moments_kernel<true><<<gl, 256, 0, d.stream>>>(
+            d.nprefix, d.nlevel, d.ncount, d.nchild, d.nfirst, d.level_info, d.pos[0], d.pos[1],
+            d.pos[2], d.mass, d.geom, d.wnode, d.spart, d.sleaf, d.next, d.noct, rf, d.err);
+    else
+        leaf_moments_kernel<false><<<gl, 256, 0, d.stream>>>(
+            d.nprefix, d.nlevel, d.ncount, d.nchild, d.nfirst, d.level_info, d.pos[0], d.pos[1],
+            d.pos[2], d.mass, d.geom, d.wnode, d.spart, d.sleaf, d.next, nullptr, rf, d.err);
+    // level kMaxLevel holds leaves only
+    for (int L = kMaxLevel - 1; L >= 0; --L) {
         const size_t bound = level_bound(L, d.n, d.node_cap);
         if (d.noct)
             tree_moments_kernel<true><<<level_grid(bound), 256, 0, d.stream>>>(
-                d.nprefix, d.nstart, d.ncount, d.nchild, d.nfirst, d.level_info, L, d.pos[0],
-                d.pos[1], d.pos[2], d.mass, d.geom, d.wnode, d.spart, d.sleaf, d.next, d.noct,
-                refit ? 1 : 0, d.err);
+                d.nprefix, d.ncount, d.nchild, d.nfirst, d.level_info, L, d.pos[0], d.pos[1],
+                d.pos[2], d.mass, d.geom, d.wnode, d.spart, d.sleaf, d.next, d.noct, rf, d.err);
         else
             tree_moments_kernel<false><<<level_grid(bound), 256, 0, d.stream>>>(
-                d.nprefix, d.nstart, d.ncount, d.nchild, d.nfirst, d.level_info, L, d.pos[0],
-                d.pos[1], d.pos[2], d.mass, d.geom, d.wnode, d.spart, d.sleaf, d.next, nullptr,
-                refit ? 1 : 0, d.err);
+                d.nprefix, d.ncount, d.nchild, d.nfirst, d.level_info, L, d.pos[0], d.pos[1],
+                d.pos[2], d.mass, d.geom, d.wnode, d.spart, d.sleaf, d.next, nullptr, rf, d.err);
     }
 }
 

@@ -56,8 +56,10 @@ def test_sorted_order_ties_by_particle_index(oracle):
     assert np.array_equal(t["sorted_ids"], order)
 
 
-@pytest.mark.parametrize("n,leaf_cap,bucket", [(1, 256, 16), (5, 256, 16), (3000, 16, 8),
+@pytest.mark.parametrize("n,leaf_cap,bucket", [(1, 256, 16), (2, 256, 16), (5, 256, 16),
+                                               (3000, 16, 8), (4096, 16, 16), (4097, 16, 3),
                                                (40000, 256, 16), (40000, 16, 1), (40000, 64, 64),
+                                               (40000, 16, 65), (40000, 16, 200),
                                                (1500000, 16, 1)])
 def test_topology_bit_exact(oracle, n, leaf_cap, bucket):
     say(f"case topology n={n} leaf_cap={leaf_cap} bucket={bucket}")
