@@ -387,12 +387,25 @@ __device__ inline void topo_tile(TopoTile& S, const unsigned long long* __restri
                                  uint32_t n, uint32_t t0, int B, int D, int lmax)
 {
     const int tid = threadIdx.x;
-    for (int m = tid; m < kTopoTile + 2 * kTopoHalo; m += kTopoThreads) {
+    // every key load of the thread in flight at once, then the digits
+    constexpr int kIters = (kTopoTile + 2 * kTopoHalo + kTopoThreads - 1) / kTopoThreads;
+    unsigned long long ka[kIters], kb[kIters];
+#pragma unroll
+    for (int k = 0; k < kIters; ++k) {
+        const int m = tid + k * kTopoThreads;
         const long long p = static_cast<long long>(t0) - kTopoHalo + m;
-        int v = -1;
-        if (p >= 1 && p < static_cast<long long>(n))
-            v = shared_digits(keys[p - 1], keys[p]);
-        S.d[m] = static_cast<int8_t>(v);
+        const bool in = m < kTopoTile + 2 * kTopoHalo && p >= 1 && p < static_cast<long long>(n);
+        ka[k] = in ? keys[p - 1] : 0ull;
+        kb[k] = in ? keys[p] : 0ull;
+    }
+#pragma unroll
+    for (int k = 0; k < kIters; ++k) {
+        const int m = tid + k * kTopoThreads;
+        const long long p = static_cast<long long>(t0) - kTopoHalo + m;
+        if (m < kTopoTile + 2 * kTopoHalo)
+            S.d[m] = static_cast<int8_t>(p >= 1 && p < static_cast<long long>(n)
+                                             ? shared_digits(ka[k], kb[k])
+                                             : -1);
     }
     for (int L = 0; L < kLevels; ++L)
         S.cnt[L][tid] = 0u;
@@ -535,14 +548,22 @@ __global__ __launch_bounds__(kTopoThreads) void topo_emit_kernel(
         S.cnt[L][4 * lane + 2] = ex + a0 + a1;
         S.cnt[L][4 * lane + 3] = ex + a0 + a1 + a2;
     }
+    // the thread's keys, loaded together (only node starts use theirs)
+    unsigned long long kq[kTopoPer];
+#pragma unroll
+    for (int q = 0; q < kTopoPer; ++q) {
+        const uint32_t i = t0 + static_cast<uint32_t>(tid * kTopoPer + q);
+        kq[q] = i < n ? keys[i] & kKeyMask : 0ull;
+    }
     __syncthreads();
+#pragma unroll
     for (int q = 0; q < kTopoPer; ++q) {
         const int r = tid * kTopoPer + q;
         const int lo = S.lo[r], hi = S.hi[r];
         if (lo > hi)
             continue;
         const uint32_t i = t0 + static_cast<uint32_t>(r);
-        const unsigned long long key = keys[i] & kKeyMask;
+        const unsigned long long key = kq[q];
         uint32_t prev = 0xffffffffu;
         for (int L = lo; L <= hi; ++L) {
             const uint32_t idx = sbase[L] + S.cnt[L][tid];
@@ -625,7 +646,7 @@ __global__ __launch_bounds__(256) void topo_link_kernel(
     const uint8_t* __restrict__ nlevel, const uint32_t* __restrict__ nstart,
     uint32_t* __restrict__ ncount, uint32_t* __restrict__ nchild,
     const uint32_t* __restrict__ nfirst, uint32_t* __restrict__ nparent,
-    const int* __restrict__ info, int* __restrict__ err)
+    uint32_t* __restrict__ sleaf, const int* __restrict__ info, int* __restrict__ err)
 {
     const int total = info[kInfoTotal];
     for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < total; x += gridDim.x * blockDim.x) {
@@ -676,6 +697,11 @@ __global__ __launch_bounds__(256) void topo_link_kernel(
             const uint32_t cc = ncount[cur];
             const uint32_t e = cc != kPending ? cs + cc : run_end(keys, n, cs, lc);
             ncount[x] = e - nstart[x];
+        }
+        if (nchild[x] == 0u) {  // a leaf: its particles point to it
+            const uint32_t s0 = nstart[x], e = min(s0 + ncount[x], n);
+            for (uint32_t q = s0; q < e; ++q)
+                sleaf[q] = static_cast<uint32_t>(x);
         }
     }
 }
@@ -733,10 +759,6 @@ __device__ inline void oct_add_child(Oct& o, const float* tc, const WalkNode& c,
     }
 }
 
-// Moments of one node. Leaves also pack their particles: spart[s] = (FP32
-// offset from the leaf lattice centre, mass) and sleaf[s] = leaf node. An
-// internal node combines its children, which must be complete. OCT: also the
-// third moments (noct, 10 per node; read by the learned far-field correction).
 // |d|^2 with explicit roundings: the moments come out the same bits in every
 // kernel that computes them, whatever the compiler fuses around them.
 __device__ inline float norm2_rn(float x, float y, float z)
@@ -744,14 +766,24 @@ __device__ inline float norm2_rn(float x, float y, float z)
     return __fmaf_rn(z, z, __fmaf_rn(y, y, __fmul_rn(x, x)));
 }
 
-template <bool OCT, bool LEAF>
+// FP64 lattice centre of a cell: the origin of its particles' FP32 offsets.
+__device__ inline void cell_origin(const int* c, const RootGeom* __restrict__ geom, double* o)
+{
+    o[0] = __fma_rn(static_cast<double>(c[0]), geom->unit_d[0], geom->lo_d[0]);
+    o[1] = __fma_rn(static_cast<double>(c[1]), geom->unit_d[1], geom->lo_d[1]);
+    o[2] = __fma_rn(static_cast<double>(c[2]), geom->unit_d[2], geom->lo_d[2]);
+}
+
+// Moments of one node. A leaf reads its particles' offsets from its lattice
+// centre through part(s, origin) (particle s, as stored in spart); an internal
+// node combines its children, which must be complete. OCT: also the third
+// moments (noct, 10 per node; read by the learned far-field correction).
+template <bool OCT, bool LEAF, class Part>
 __device__ inline void node_moments(
     int node, int level, const uint64_t* __restrict__ nprefix, const uint32_t* __restrict__ ncount,
     const uint32_t* __restrict__ nchild, const uint32_t* __restrict__ nfirst,
-    const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
-    const float* __restrict__ pm, const RootGeom* __restrict__ geom, WalkNode* __restrict__ wn,
-    float4* __restrict__ spart, uint32_t* __restrict__ sleaf, float* __restrict__ ext,
-    float* __restrict__ noct, int refit, int* err)
+    const RootGeom* __restrict__ geom, WalkNode* __restrict__ wn, float* __restrict__ ext,
+    float* __restrict__ noct, int refit, int* err, Part part)
 {
     const float ux = geom->unit_f[0], uy = geom->unit_f[1], uz = geom->unit_f[2];
     {
@@ -771,18 +803,12 @@ __device__ inline void node_moments(
             oct_zero(oc);
         if (LEAF) {
             count = ncount[node];
-            const double ox = __fma_rn(static_cast<double>(c[0]), geom->unit_d[0], geom->lo_d[0]);
-            const double oy = __fma_rn(static_cast<double>(c[1]), geom->unit_d[1], geom->lo_d[1]);
-            const double oz = __fma_rn(static_cast<double>(c[2]), geom->unit_d[2], geom->lo_d[2]);
+            double o[3];
+            cell_origin(c, geom, o);
             float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-#pragma unroll 4
             for (uint32_t s = first; s < first + count; ++s) {
-                const float x = static_cast<float>(px[s] - ox);
-                const float y = static_cast<float>(py[s] - oy);
-                const float z = static_cast<float>(pz[s] - oz);
-                const float m = pm[s];
-                spart[s] = make_float4(x, y, z, m);
-                sleaf[s] = static_cast<uint32_t>(node);
+                const float4 v = part(s, o);
+                const float x = v.x, y = v.y, z = v.z, m = v.w;
                 e = fmaxf(e, fmaxf(fabsf(x) / ux, fmaxf(fabsf(y) / uy, fabsf(z) / uz)));
                 M = __fadd_rn(M, m);
                 sx = __fmaf_rn(m, x, sx);
@@ -794,9 +820,8 @@ __device__ inline void node_moments(
                 cy = sy / M;
                 cz = sz / M;
             }
-#pragma unroll 4
             for (uint32_t s = first; s < first + count; ++s) {
-                const float4 p = spart[s];
+                const float4 p = part(s, o);
                 const float dx = __fsub_rn(p.x, cx), dy = __fsub_rn(p.y, cy),
                             dz = __fsub_rn(p.z, cz);
                 const float r2 = norm2_rn(dx, dy, dz);
@@ -917,24 +942,77 @@ __device__ inline void node_moments(
     }
 }
 
-// Leaves of every level in one launch: a leaf depends only on its particles.
+// Leaves of every level, one thread per particle in chunks of kLeafChunk:
+// each particle stores its offset from its leaf's lattice centre (spart,
+// coalesced) and the thread of a leaf's first particle then sums the leaf,
+// reading the chunk's offsets from LDS and recomputing those past the chunk
+// (the same arithmetic, so the same bits). sleaf must be set.
+constexpr int kLeafChunk = 256;
+
 template <bool OCT>
-__global__ __launch_bounds__(256) void leaf_moments_kernel(
+__global__ __launch_bounds__(kLeafChunk) void leaf_moments_kernel(
     const uint64_t* __restrict__ nprefix, const uint8_t* __restrict__ nlevel,
-    const uint32_t* __restrict__ ncount, const uint32_t* __restrict__ nchild,
-    const uint32_t* __restrict__ nfirst, const int* __restrict__ info,
+    const uint32_t* __restrict__ nstart, const uint32_t* __restrict__ ncount,
+    const uint32_t* __restrict__ nchild, const uint32_t* __restrict__ nfirst,
+    const uint32_t* __restrict__ sleaf, uint32_t n, const int* __restrict__ info,
     const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
     const float* __restrict__ pm, const RootGeom* __restrict__ geom, WalkNode* __restrict__ wn,
-    float4* __restrict__ spart, uint32_t* __restrict__ sleaf, float* __restrict__ ext,
-    float* __restrict__ noct, int refit, int* err)
+    float4* __restrict__ spart, float* __restrict__ ext, float* __restrict__ noct, int refit,
+    int* err)
+{
+    __shared__ float4 sp[kLeafChunk];
+    const uint32_t b0 = blockIdx.x * kLeafChunk;
+    const uint32_t p = b0 + threadIdx.x;
+    const uint32_t total = static_cast<uint32_t>(info[kInfoTotal]);
+    uint32_t leaf = 0;
+    int level = 0;
+    bool first = false;
+    if (p < n) {
+        leaf = sleaf[p];
+        if (leaf < total) {
+            level = nlevel[leaf];
+            int c[3];
+            lattice_centre(nprefix[leaf], level, c);
+            double o[3];
+            cell_origin(c, geom, o);
+            const float4 v =
+                make_float4(static_cast<float>(px[p] - o[0]), static_cast<float>(py[p] - o[1]),
+                            static_cast<float>(pz[p] - o[2]), pm[p]);
+            spart[p] = v;
+            sp[threadIdx.x] = v;
+            first = nstart[leaf] == p;
+        } else {
+            atomicOr(err, kErrTopology);
+        }
+    }
+    __syncthreads();
+    if (!first)
+        return;
+    const uint32_t lim = b0 + kLeafChunk;
+    node_moments<OCT, true>(
+        static_cast<int>(leaf), level, nprefix, ncount, nchild, nfirst, geom, wn, ext, noct, refit,
+        err, [&](uint32_t s, const double* o) {
+            if (s < lim)
+                return sp[s - b0];
+            return make_float4(static_cast<float>(px[s] - o[0]), static_cast<float>(py[s] - o[1]),
+                               static_cast<float>(pz[s] - o[2]), pm[s]);
+        });
+}
+
+// sleaf[s] = the leaf holding particle s (topology built level by level).
+__global__ __launch_bounds__(256) void leaf_fill_kernel(const uint32_t* __restrict__ nstart,
+                                                        const uint32_t* __restrict__ ncount,
+                                                        const uint32_t* __restrict__ nchild,
+                                                        const int* __restrict__ info, uint32_t n,
+                                                        uint32_t* __restrict__ sleaf)
 {
     const int total = info[kInfoTotal];
-    for (int node = blockIdx.x * blockDim.x + threadIdx.x; node < total;
-         node += gridDim.x * blockDim.x) {
-        if (nchild[node] != 0)
+    for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < total; x += gridDim.x * blockDim.x) {
+        if (nchild[x] != 0u)
             continue;
-        node_moments<OCT, true>(node, nlevel[node], nprefix, ncount, nchild, nfirst, px, py, pz,
-                                pm, geom, wn, spart, sleaf, ext, noct, refit, err);
+        const uint32_t s0 = nstart[x], e = min(s0 + ncount[x], n);
+        for (uint32_t q = s0; q < e; ++q)
+            sleaf[q] = static_cast<uint32_t>(x);
     }
 }
 
@@ -943,10 +1021,8 @@ template <bool OCT>
 __global__ __launch_bounds__(256) void tree_moments_kernel(
     const uint64_t* __restrict__ nprefix, const uint32_t* __restrict__ ncount,
     const uint32_t* __restrict__ nchild, const uint32_t* __restrict__ nfirst,
-    const int* __restrict__ info, int level, const double* __restrict__ px,
-    const double* __restrict__ py, const double* __restrict__ pz, const float* __restrict__ pm,
-    const RootGeom* __restrict__ geom, WalkNode* __restrict__ wn, float4* __restrict__ spart,
-    uint32_t* __restrict__ sleaf, float* __restrict__ ext, float* __restrict__ noct, int refit,
+    const int* __restrict__ info, int level, const RootGeom* __restrict__ geom,
+    WalkNode* __restrict__ wn, float* __restrict__ ext, float* __restrict__ noct, int refit,
     int* err)
 {
     const int base = info[level];
@@ -955,8 +1031,8 @@ __global__ __launch_bounds__(256) void tree_moments_kernel(
         const int node = base + k;
         if (nchild[node] == 0)
             continue;
-        node_moments<OCT, false>(node, level, nprefix, ncount, nchild, nfirst, px, py, pz, pm,
-                                 geom, wn, spart, sleaf, ext, noct, refit, err);
+        node_moments<OCT, false>(node, level, nprefix, ncount, nchild, nfirst, geom, wn, ext, noct,
+                                 refit, err, [](uint32_t, const double*) { return float4{}; });
     }
 }
 
@@ -1102,7 +1178,8 @@ void launch_tree_build(Device& d, int leaf_depth, int bucket)
             d.err);
         topo_link_kernel<<<level_grid(std::min<size_t>(d.node_cap, 2 * d.n + 64)), 256, 0,
                            d.stream>>>(sorted, n, d.nprefix, d.nlevel, d.nstart, d.ncount,
-                                       d.nchild, d.nfirst, d.nparent, d.level_info, d.err);
+                                       d.nchild, d.nfirst, d.nparent, d.sleaf, d.level_info,
+                                       d.err);
         return;
     }
     // Buckets wider than the tiles' halo: level by level, top down.
@@ -1128,6 +1205,8 @@ void launch_tree_build(Device& d, int leaf_depth, int bucket)
             d.nprefix, d.nlevel, d.nstart, d.ncount, d.nchild, d.nfirst, d.nparent, d.scan_tmp,
             d.bnd_tmp, d.level_info, L);
     }
+    leaf_fill_kernel<<<level_grid(std::min<size_t>(d.node_cap, 2 * d.n + 64)), 256, 0,
+                       d.stream>>>(d.nstart, d.ncount, d.nchild, d.level_info, n, d.sleaf);
 }
 
 void launch_groups(Device& d)
@@ -1144,26 +1223,31 @@ void launch_groups(Device& d)
 void launch_tree_moments(Device& d, bool refit)
 {
     const int rf = refit ? 1 : 0;
-    const unsigned gl = level_grid(std::min<size_t>(d.node_cap, d.n));
-    if (d.noct)
-        leaf_moments_kernel<true><<<gl, 256, 0, d.stream>>>(
-            d.nprefix, d.nlevel, d.ncount, d.nchild, d.nfirst, d.level_info, d.pos[0], d.pos[1],
-            d.pos[2], d.mass, d.geom, d.wnode, d.spart, d.sleaf, d.next, d.noct, rf, d.err);
-    else
-        leaf_moments_kernel<false><<<gl, 256, 0, d.stream>>>(
-            d.nprefix, d.nlevel, d.ncount, d.nchild, d.nfirst, d.level_info, d.pos[0], d.pos[1],
-            d.pos[2], d.mass, d.geom, d.wnode, d.spart, d.sleaf, d.next, nullptr, rf, d.err);
+    const uint32_t n = static_cast<uint32_t>(d.n);
+    const unsigned gl = static_cast<unsigned>((d.n + kLeafChunk - 1) / kLeafChunk);
+    if (gl > 0) {
+        if (d.noct)
+            leaf_moments_kernel<true><<<gl, kLeafChunk, 0, d.stream>>>(
+                d.nprefix, d.nlevel, d.nstart, d.ncount, d.nchild, d.nfirst, d.sleaf, n,
+                d.level_info, d.pos[0], d.pos[1], d.pos[2], d.mass, d.geom, d.wnode, d.spart,
+                d.next, d.noct, rf, d.err);
+        else
+            leaf_moments_kernel<false><<<gl, kLeafChunk, 0, d.stream>>>(
+                d.nprefix, d.nlevel, d.nstart, d.ncount, d.nchild, d.nfirst, d.sleaf, n,
+                d.level_info, d.pos[0], d.pos[1], d.pos[2], d.mass, d.geom, d.wnode, d.spart,
+                d.next, nullptr, rf, d.err);
+    }
     // level kMaxLevel holds leaves only
     for (int L = kMaxLevel - 1; L >= 0; --L) {
         const size_t bound = level_bound(L, d.n, d.node_cap);
         if (d.noct)
             tree_moments_kernel<true><<<level_grid(bound), 256, 0, d.stream>>>(
-                d.nprefix, d.ncount, d.nchild, d.nfirst, d.level_info, L, d.pos[0], d.pos[1],
-                d.pos[2], d.mass, d.geom, d.wnode, d.spart, d.sleaf, d.next, d.noct, rf, d.err);
+                d.nprefix, d.ncount, d.nchild, d.nfirst, d.level_info, L, d.geom, d.wnode, d.next,
+                d.noct, rf, d.err);
         else
             tree_moments_kernel<false><<<level_grid(bound), 256, 0, d.stream>>>(
-                d.nprefix, d.ncount, d.nchild, d.nfirst, d.level_info, L, d.pos[0], d.pos[1],
-                d.pos[2], d.mass, d.geom, d.wnode, d.spart, d.sleaf, d.next, nullptr, rf, d.err);
+                d.nprefix, d.ncount, d.nchild, d.nfirst, d.level_info, L, d.geom, d.wnode, d.next,
+                nullptr, rf, d.err);
     }
 }
 
