@@ -376,17 +376,37 @@ __device__ inline int shared_digits(unsigned long long a, unsigned long long b)
 struct TopoTile {
     int8_t d[kTopoTile + 2 * kTopoHalo];  // d at particle t0 - H + m (-1 outside [1, n))
     int8_t w[kTopoTile + kTopoHalo];      // w at window start t0 - H + m
-    int8_t lo[kTopoTile];
-    int8_t hi[kTopoTile];
     uint32_t cnt[kLevels][kTopoThreads];  // nodes per level and thread
 };
 
-// d, w, the level range [lo, hi] of every particle of the tile starting at t0
-// (levels above lmax dropped) and each thread's node count per level.
-__device__ inline void topo_tile(TopoTile& S, const unsigned long long* __restrict__ keys,
-                                 uint32_t n, uint32_t t0, int B, int D, int lmax)
+// Per-particle level range [lo, hi] and, when the node at hi holds at most B
+// particles, its count c (else 0): lo | (hi + 1) << 8 | c << 16.
+__device__ inline uint32_t topo_pack(int lo, int hi, int c)
 {
+    return static_cast<uint32_t>(lo) | (static_cast<uint32_t>(hi + 1) << 8) |
+           (static_cast<uint32_t>(c) << 16);
+}
+__device__ inline int topo_lo(uint32_t v) { return static_cast<int>(v & 0xffu); }
+__device__ inline int topo_hi(uint32_t v) { return static_cast<int>((v >> 8) & 0xffu) - 1; }
+__device__ inline int topo_c(uint32_t v) { return static_cast<int>((v >> 16) & 0xffu); }
+
+__device__ inline uint32_t wave_total(uint32_t v)
+{
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Per tile: the level range of every particle (pinfo) and the nodes per
+// level (tilecnt[L * ntiles + tile]).
+__global__ __launch_bounds__(kTopoThreads) void topo_count_kernel(
+    const unsigned long long* __restrict__ keys, uint32_t n, int B, int D,
+    uint32_t* __restrict__ pinfo, uint32_t* __restrict__ tilecnt, uint32_t ntiles)
+{
+    __shared__ TopoTile S;
     const int tid = threadIdx.x;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t t0 = tile * kTopoTile;
     // every key load of the thread in flight at once, then the digits
     constexpr int kIters = (kTopoTile + 2 * kTopoHalo + kTopoThreads - 1) / kTopoThreads;
     unsigned long long ka[kIters], kb[kIters];
@@ -417,41 +437,38 @@ __device__ inline void topo_tile(TopoTile& S, const unsigned long long* __restri
         S.w[m] = static_cast<int8_t>(v);
     }
     __syncthreads();
+    uint32_t pk[kTopoPer];
+#pragma unroll
     for (int q = 0; q < kTopoPer; ++q) {
         const int r = tid * kTopoPer + q;
         const uint32_t i = t0 + static_cast<uint32_t>(r);
-        int lo = kLevels, hi = -1;
+        int lo = kLevels, hi = -1, c = 0;
         if (i < n) {
             int W = -1;
             for (int b = 0; b <= B; ++b)  // r + H - b >= 0
                 W = max(W, static_cast<int>(S.w[r + kTopoHalo - b]));
             lo = i == 0 ? 0 : static_cast<int>(S.d[r + kTopoHalo]) + 1;
-            hi = min(min(kMaxLevel, max(D, W + 1)), lmax);
+            hi = min(kMaxLevel, max(D, W + 1));
+            // the leaf at hi ends at the next run start within B, if any
+            // (d is -1 at and beyond n)
+            for (int b = 1; b <= B && lo <= hi; ++b) {
+                if (S.d[r + kTopoHalo + b] < hi) {
+                    c = b;
+                    break;
+                }
+            }
         }
-        S.lo[r] = static_cast<int8_t>(lo);
-        S.hi[r] = static_cast<int8_t>(hi);
+        pk[q] = topo_pack(lo, hi, c);
         for (int L = lo; L <= hi; ++L)
             S.cnt[L][tid] += 1u;
     }
+    const uint32_t i0 = t0 + static_cast<uint32_t>(tid * kTopoPer);
+#pragma unroll
+    for (int q = 0; q < kTopoPer; ++q)
+        if (i0 + q < n)
+            pinfo[i0 + q] = pk[q];
     __syncthreads();
-}
-
-__device__ inline uint32_t wave_total(uint32_t v)
-{
-    for (int o = 32; o > 0; o >>= 1)
-        v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-// Nodes per level in each tile: tilecnt[L * ntiles + tile].
-__global__ __launch_bounds__(kTopoThreads) void topo_count_kernel(
-    const unsigned long long* __restrict__ keys, uint32_t n, int B, int D,
-    uint32_t* __restrict__ tilecnt, uint32_t ntiles)
-{
-    __shared__ TopoTile S;
-    const uint32_t tile = blockIdx.x;
-    topo_tile(S, keys, n, tile * kTopoTile, B, D, kMaxLevel);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = tid & 63, wv = tid >> 6;
     for (int L = wv; L < kLevels; L += kTopoThreads / 64) {
         const uint32_t v = wave_total(S.cnt[L][lane] + S.cnt[L][lane + 64] +
                                       S.cnt[L][lane + 128] + S.cnt[L][lane + 192]);
@@ -482,24 +499,35 @@ __global__ __launch_bounds__(kScanBlock) void topo_scan_kernel(uint32_t* __restr
         tilecnt[static_cast<size_t>(kLevels) * ntiles + L] = carry;
 }
 
-// The nodes that start in one tile, in breadth-first place. The first level
-// whose nodes would not fit in node_cap is dropped with every deeper one (the
-// nodes above it that would split stay leaves; flagged). Block 0 also writes
-// the level table.
+// The nodes that start in one tile, in breadth-first place, level by level
+// (a thread's nodes of one level take consecutive slots, so neighbouring
+// lanes write neighbouring slots). The first level whose nodes would not fit
+// in node_cap is dropped with every deeper one (the nodes above it that
+// would split stay leaves; flagged). Block 0 also writes the level table.
 __global__ __launch_bounds__(kTopoThreads) void topo_emit_kernel(
-    const unsigned long long* __restrict__ keys, uint32_t n, int B, int D,
+    const unsigned long long* __restrict__ keys, uint32_t n, const uint32_t* __restrict__ pinfo,
     const uint32_t* __restrict__ tilecnt, uint32_t ntiles, int node_cap,
     uint64_t* __restrict__ nprefix, uint8_t* __restrict__ nlevel, uint32_t* __restrict__ nstart,
     uint32_t* __restrict__ ncount, uint32_t* __restrict__ nchild, uint32_t* __restrict__ nfirst,
     uint32_t* __restrict__ nparent, int* __restrict__ info, int* __restrict__ err)
 {
-    __shared__ TopoTile S;
+    __shared__ uint32_t cnt[kLevels][kTopoThreads];
     __shared__ uint32_t sbase[kLevels];  // level base + this tile's offset
     __shared__ int slmax;
     const int tid = threadIdx.x;
     const uint32_t tile = blockIdx.x;
     const uint32_t t0 = tile * kTopoTile;
+    const uint32_t i0 = t0 + static_cast<uint32_t>(tid * kTopoPer);
     const uint32_t* tot = tilecnt + static_cast<size_t>(kLevels) * ntiles;
+    // the thread's level ranges and keys, loaded together
+    uint32_t pk[kTopoPer];
+    unsigned long long kq[kTopoPer];
+#pragma unroll
+    for (int q = 0; q < kTopoPer; ++q) {
+        const bool in = i0 + q < n;
+        pk[q] = in ? pinfo[i0 + q] : topo_pack(kLevels, -1, 0);
+        kq[q] = in ? keys[i0 + q] & kKeyMask : 0ull;
+    }
     if (tid == 0)
         slmax = kMaxLevel;
     __syncthreads();
@@ -535,63 +563,71 @@ __global__ __launch_bounds__(kTopoThreads) void topo_emit_kernel(
             }
         }
     }
-    topo_tile(S, keys, n, t0, B, D, lmax);  // begins and ends with a barrier-separated phase
+    // levels past lmax dropped: the node there becomes a leaf of unknown count
+    int lmin_t = kLevels, lmax_t = -1;
+#pragma unroll
+    for (int q = 0; q < kTopoPer; ++q) {
+        const int lo = topo_lo(pk[q]), hi = topo_hi(pk[q]);
+        if (hi > lmax)
+            pk[q] = topo_pack(lo, lmax, 0);
+        if (lo <= min(hi, lmax)) {
+            lmin_t = min(lmin_t, lo);
+            lmax_t = max(lmax_t, min(hi, lmax));
+        }
+    }
+    for (int L = 0; L < kLevels; ++L)
+        cnt[L][tid] = 0u;
+#pragma unroll
+    for (int q = 0; q < kTopoPer; ++q)
+        for (int L = topo_lo(pk[q]); L <= topo_hi(pk[q]); ++L)
+            cnt[L][tid] += 1u;
+    __syncthreads();
     // per level: exclusive scan of the thread counts, in place
     const int lane = tid & 63, wv = tid >> 6;
     for (int L = wv; L < kLevels; L += kTopoThreads / 64) {
-        const uint32_t a0 = S.cnt[L][4 * lane], a1 = S.cnt[L][4 * lane + 1];
-        const uint32_t a2 = S.cnt[L][4 * lane + 2], a3 = S.cnt[L][4 * lane + 3];
+        const uint32_t a0 = cnt[L][4 * lane], a1 = cnt[L][4 * lane + 1];
+        const uint32_t a2 = cnt[L][4 * lane + 2], a3 = cnt[L][4 * lane + 3];
         const uint32_t sum = a0 + a1 + a2 + a3;
         const uint32_t ex = wave_inclusive(sum) - sum;
-        S.cnt[L][4 * lane] = ex;
-        S.cnt[L][4 * lane + 1] = ex + a0;
-        S.cnt[L][4 * lane + 2] = ex + a0 + a1;
-        S.cnt[L][4 * lane + 3] = ex + a0 + a1 + a2;
-    }
-    // the thread's keys, loaded together (only node starts use theirs)
-    unsigned long long kq[kTopoPer];
-#pragma unroll
-    for (int q = 0; q < kTopoPer; ++q) {
-        const uint32_t i = t0 + static_cast<uint32_t>(tid * kTopoPer + q);
-        kq[q] = i < n ? keys[i] & kKeyMask : 0ull;
+        cnt[L][4 * lane] = ex;
+        cnt[L][4 * lane + 1] = ex + a0;
+        cnt[L][4 * lane + 2] = ex + a0 + a1;
+        cnt[L][4 * lane + 3] = ex + a0 + a1 + a2;
     }
     __syncthreads();
+    uint32_t prev[kTopoPer];  // particle q's node at the level before
 #pragma unroll
-    for (int q = 0; q < kTopoPer; ++q) {
-        const int r = tid * kTopoPer + q;
-        const int lo = S.lo[r], hi = S.hi[r];
-        if (lo > hi)
-            continue;
-        const uint32_t i = t0 + static_cast<uint32_t>(r);
-        const unsigned long long key = kq[q];
-        uint32_t prev = 0xffffffffu;
-        for (int L = lo; L <= hi; ++L) {
-            const uint32_t idx = sbase[L] + S.cnt[L][tid];
-            S.cnt[L][tid] += 1u;
-            nprefix[idx] = L == 0 ? 0ull : key >> (3 * (kMaxLevel - L));
+    for (int q = 0; q < kTopoPer; ++q)
+        prev[q] = 0xffffffffu;
+    for (int L = lmin_t; L <= lmax_t; ++L) {
+        uint32_t idx = sbase[L] + cnt[L][tid];
+        const int sh = 3 * (kMaxLevel - L);
+#pragma unroll
+        for (int q = 0; q < kTopoPer; ++q) {
+            const int lo = topo_lo(pk[q]), hi = topo_hi(pk[q]);
+            if (L < lo || L > hi)
+                continue;
+            const uint32_t i = i0 + static_cast<uint32_t>(q);
+            nprefix[idx] = L == 0 ? 0ull : kq[q] >> sh;
             nlevel[idx] = static_cast<uint8_t>(L);
             nstart[idx] = i;
-            if (L < hi) {
-                // the first child starts here too: its rank at L + 1
-                nfirst[idx] = sbase[L + 1] + S.cnt[L + 1][tid];
+            if (L < hi) {  // nfirst: written by the first child, next level
                 nchild[idx] = kPending;
                 ncount[idx] = kPending;
             } else {
                 nfirst[idx] = i;
                 nchild[idx] = 0u;
-                // a leaf below level 21 that was not cut by lmax holds at most
-                // B particles: its end is the next run start within B
-                uint32_t c = kPending;
-                for (int b = 1; b <= B; ++b) {
-                    if (S.d[r + kTopoHalo + b] < L) {  // -1 at and beyond n
-                        c = static_cast<uint32_t>(b);
-                        break;
-                    }
-                }
-                ncount[idx] = c;
+                const int c = topo_c(pk[q]);
+                ncount[idx] = c > 0 ? static_cast<uint32_t>(c) : kPending;
             }
-            nparent[idx] = L == 0 ? 0xffffffffu : (L > lo ? prev : kPending);
-            prev = idx;
+            if (L > lo) {  // the first child of the node above
+                nparent[idx] = prev[q];
+                nfirst[prev[q]] = idx;
+            } else {
+                nparent[idx] = L == 0 ? 0xffffffffu : kPending;
+            }
+            prev[q] = idx;
+            ++idx;
         }
     }
 }
@@ -1169,13 +1205,13 @@ void launch_tree_build(Device& d, int leaf_depth, int bucket)
     const unsigned long long* sorted = reinterpret_cast<const unsigned long long*>(d.keys);
     if (n > 0 && bucket <= kTopoHalo && d.topo_cnt) {
         const uint32_t ntiles = topo_tiles(d.n);
+        uint32_t* pinfo = d.bnd_tmp;  // n words (the level-by-level build's child bounds)
         topo_count_kernel<<<ntiles, kTopoThreads, 0, d.stream>>>(sorted, n, bucket, leaf_depth,
-                                                                 d.topo_cnt, ntiles);
+                                                                 pinfo, d.topo_cnt, ntiles);
         topo_scan_kernel<<<kLevels, kScanBlock, 0, d.stream>>>(d.topo_cnt, ntiles);
         topo_emit_kernel<<<ntiles, kTopoThreads, 0, d.stream>>>(
-            sorted, n, bucket, leaf_depth, d.topo_cnt, ntiles, static_cast<int>(d.node_cap),
-            d.nprefix, d.nlevel, d.nstart, d.ncount, d.nchild, d.nfirst, d.nparent, d.level_info,
-            d.err);
+            sorted, n, pinfo, d.topo_cnt, ntiles, static_cast<int>(d.node_cap), d.nprefix,
+            d.nlevel, d.nstart, d.ncount, d.nchild, d.nfirst, d.nparent, d.level_info, d.err);
         topo_link_kernel<<<level_grid(std::min<size_t>(d.node_cap, 2 * d.n + 64)), 256, 0,
                            d.stream>>>(sorted, n, d.nprefix, d.nlevel, d.nstart, d.ncount,
                                        d.nchild, d.nfirst, d.nparent, d.sleaf, d.level_info,
