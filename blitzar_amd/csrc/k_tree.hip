@@ -437,10 +437,11 @@ __global__ __launch_bounds__(kTopoThreads) void topo_count_kernel(
         S.w[m] = static_cast<int8_t>(v);
     }
     __syncthreads();
-    uint32_t pk[kTopoPer];
-#pragma unroll
+    // particles interleaved over the threads (neighbouring lanes read
+    // neighbouring bytes of d and w, and store neighbouring words)
+#pragma unroll 4
     for (int q = 0; q < kTopoPer; ++q) {
-        const int r = tid * kTopoPer + q;
+        const int r = q * kTopoThreads + tid;
         const uint32_t i = t0 + static_cast<uint32_t>(r);
         int lo = kLevels, hi = -1, c = 0;
         if (i < n) {
@@ -458,15 +459,11 @@ __global__ __launch_bounds__(kTopoThreads) void topo_count_kernel(
                 }
             }
         }
-        pk[q] = topo_pack(lo, hi, c);
+        if (i < n)
+            pinfo[i] = topo_pack(lo, hi, c);
         for (int L = lo; L <= hi; ++L)
             S.cnt[L][tid] += 1u;
     }
-    const uint32_t i0 = t0 + static_cast<uint32_t>(tid * kTopoPer);
-#pragma unroll
-    for (int q = 0; q < kTopoPer; ++q)
-        if (i0 + q < n)
-            pinfo[i0 + q] = pk[q];
     __syncthreads();
     const int lane = tid & 63, wv = tid >> 6;
     for (int L = wv; L < kLevels; L += kTopoThreads / 64) {
@@ -522,11 +519,30 @@ __global__ __launch_bounds__(kTopoThreads) void topo_emit_kernel(
     // the thread's level ranges and keys, loaded together
     uint32_t pk[kTopoPer];
     unsigned long long kq[kTopoPer];
+    if (i0 + kTopoPer <= n) {
+        const uint4* in4 = reinterpret_cast<const uint4*>(pinfo + i0);
+        const ulonglong2* k2 = reinterpret_cast<const ulonglong2*>(keys + i0);
 #pragma unroll
-    for (int q = 0; q < kTopoPer; ++q) {
-        const bool in = i0 + q < n;
-        pk[q] = in ? pinfo[i0 + q] : topo_pack(kLevels, -1, 0);
-        kq[q] = in ? keys[i0 + q] & kKeyMask : 0ull;
+        for (int k = 0; k < kTopoPer / 4; ++k) {
+            const uint4 v = in4[k];
+            pk[4 * k] = v.x;
+            pk[4 * k + 1] = v.y;
+            pk[4 * k + 2] = v.z;
+            pk[4 * k + 3] = v.w;
+        }
+#pragma unroll
+        for (int k = 0; k < kTopoPer / 2; ++k) {
+            const ulonglong2 v = k2[k];
+            kq[2 * k] = v.x & kKeyMask;
+            kq[2 * k + 1] = v.y & kKeyMask;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < kTopoPer; ++q) {
+            const bool in = i0 + q < n;
+            pk[q] = in ? pinfo[i0 + q] : topo_pack(kLevels, -1, 0);
+            kq[q] = in ? keys[i0 + q] & kKeyMask : 0ull;
+        }
     }
     if (tid == 0)
         slmax = kMaxLevel;
@@ -829,9 +845,10 @@ __device__ inline void node_moments(
         const uint32_t first = nfirst[node];
         float M = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
         float qxx = 0.0f, qxy = 0.0f, qxz = 0.0f, qyy = 0.0f, qyz = 0.0f, bmax = 0.0f, trs = 0.0f;
-        // e: half-width, in lattice units along the widest axis, of the cube
-        // about the lattice centre that holds the subtree's particles (the
-        // cell's own half-width 2^(21 - level) unless they left the cell)
+        // e (refit only): half-width, in lattice units along the widest
+        // axis, of the cube about the lattice centre that holds the subtree's
+        // particles (the cell's own half-width 2^(21 - level) unless they
+        // left the cell)
         float e = 0.0f;
         uint32_t count = 0;
         Oct oc;
@@ -845,7 +862,8 @@ __device__ inline void node_moments(
             for (uint32_t s = first; s < first + count; ++s) {
                 const float4 v = part(s, o);
                 const float x = v.x, y = v.y, z = v.z, m = v.w;
-                e = fmaxf(e, fmaxf(fabsf(x) / ux, fmaxf(fabsf(y) / uy, fabsf(z) / uz)));
+                if (refit)  // the extent matters only to a refit
+                    e = fmaxf(e, fmaxf(fabsf(x) / ux, fmaxf(fabsf(y) / uy, fabsf(z) / uz)));
                 M = __fadd_rn(M, m);
                 sx = __fmaf_rn(m, x, sx);
                 sy = __fmaf_rn(m, y, sy);
@@ -917,15 +935,19 @@ __device__ inline void node_moments(
                 if (live && m > 0.0f)
                     bmax = fmaxf(bmax, __fadd_rn(sqrtf(r2), ch.bmax));
                 if (live) {
-                    const int off = max(abs(ch.cx - c[0]), max(abs(ch.cy - c[1]), abs(ch.cz - c[2])));
-                    e = fmaxf(e, ext[first + jj] + static_cast<float>(off));
+                    if (refit) {
+                        const int off =
+                            max(abs(ch.cx - c[0]), max(abs(ch.cy - c[1]), abs(ch.cz - c[2])));
+                        e = fmaxf(e, ext[first + jj] + static_cast<float>(off));
+                    }
                     if (OCT)
                         oct_add_child(oc, noct + static_cast<size_t>(first + jj) * 10, ch, dx, dy,
                                       dz);
                 }
             }
         }
-        ext[node] = e;
+        if (refit)
+            ext[node] = e;
         if (OCT) {
             float* o = noct + static_cast<size_t>(node) * 10;
 #pragma unroll
@@ -978,15 +1000,18 @@ __device__ inline void node_moments(
     }
 }
 
-// Leaves of every level, one thread per particle in chunks of kLeafChunk:
-// each particle stores its offset from its leaf's lattice centre (spart,
-// coalesced) and the thread of a leaf's first particle then sums the leaf,
+// Leaves of every level. Particles in chunks of kLeafChunk (kLeafPer per
+// thread, coalesced): each stores its offset from its leaf's lattice centre
+// (spart) and keeps it in LDS. The chunk's leaf starts are then listed and
+// summed by consecutive threads (full wavefronts, not one lane in six),
 // reading the chunk's offsets from LDS and recomputing those past the chunk
 // (the same arithmetic, so the same bits). sleaf must be set.
-constexpr int kLeafChunk = 256;
+constexpr int kLeafThreads = 256;
+constexpr int kLeafPer = 4;
+constexpr int kLeafChunk = kLeafThreads * kLeafPer;
 
 template <bool OCT>
-__global__ __launch_bounds__(kLeafChunk) void leaf_moments_kernel(
+__global__ __launch_bounds__(kLeafThreads) void leaf_moments_kernel(
     const uint64_t* __restrict__ nprefix, const uint8_t* __restrict__ nlevel,
     const uint32_t* __restrict__ nstart, const uint32_t* __restrict__ ncount,
     const uint32_t* __restrict__ nchild, const uint32_t* __restrict__ nfirst,
@@ -997,42 +1022,61 @@ __global__ __launch_bounds__(kLeafChunk) void leaf_moments_kernel(
     int* err)
 {
     __shared__ float4 sp[kLeafChunk];
+    __shared__ uint32_t starts[kLeafChunk];  // chunk-relative first particle of each leaf
+    __shared__ uint32_t nstarts;
     const uint32_t b0 = blockIdx.x * kLeafChunk;
-    const uint32_t p = b0 + threadIdx.x;
     const uint32_t total = static_cast<uint32_t>(info[kInfoTotal]);
-    uint32_t leaf = 0;
-    int level = 0;
-    bool first = false;
-    if (p < n) {
-        leaf = sleaf[p];
-        if (leaf < total) {
-            level = nlevel[leaf];
-            int c[3];
-            lattice_centre(nprefix[leaf], level, c);
-            double o[3];
-            cell_origin(c, geom, o);
-            const float4 v =
-                make_float4(static_cast<float>(px[p] - o[0]), static_cast<float>(py[p] - o[1]),
-                            static_cast<float>(pz[p] - o[2]), pm[p]);
-            spart[p] = v;
-            sp[threadIdx.x] = v;
-            first = nstart[leaf] == p;
-        } else {
-            atomicOr(err, kErrTopology);
+    if (threadIdx.x == 0)
+        nstarts = 0u;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kLeafPer; ++k) {
+        const uint32_t r = static_cast<uint32_t>(k * kLeafThreads) + threadIdx.x;
+        const uint32_t p = b0 + r;
+        bool first = false;
+        if (p < n) {
+            const uint32_t leaf = sleaf[p];
+            if (leaf < total) {
+                int c[3];
+                lattice_centre(nprefix[leaf], nlevel[leaf], c);
+                double o[3];
+                cell_origin(c, geom, o);
+                const float4 v = make_float4(
+                    static_cast<float>(px[p] - o[0]), static_cast<float>(py[p] - o[1]),
+                    static_cast<float>(pz[p] - o[2]), pm[p]);
+                spart[p] = v;
+                sp[r] = v;
+                first = nstart[leaf] == p;
+            } else {
+                atomicOr(err, kErrTopology);
+            }
         }
+        // list the leaf starts (order within the list is free)
+        const unsigned long long mask = __ballot(first);
+        const int lane = threadIdx.x & 63;
+        uint32_t base = 0;
+        if (lane == 0 && mask)
+            base = atomicAdd(&nstarts, static_cast<uint32_t>(__popcll(mask)));
+        base = __shfl(base, 0, 64);
+        if (first)
+            starts[base + static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)))] = r;
     }
     __syncthreads();
-    if (!first)
-        return;
+    const uint32_t ns = nstarts;
     const uint32_t lim = b0 + kLeafChunk;
-    node_moments<OCT, true>(
-        static_cast<int>(leaf), level, nprefix, ncount, nchild, nfirst, geom, wn, ext, noct, refit,
-        err, [&](uint32_t s, const double* o) {
-            if (s < lim)
-                return sp[s - b0];
-            return make_float4(static_cast<float>(px[s] - o[0]), static_cast<float>(py[s] - o[1]),
-                               static_cast<float>(pz[s] - o[2]), pm[s]);
-        });
+    for (uint32_t t = threadIdx.x; t < ns; t += kLeafThreads) {
+        const uint32_t p = b0 + starts[t];
+        const uint32_t leaf = sleaf[p];
+        node_moments<OCT, true>(
+            static_cast<int>(leaf), nlevel[leaf], nprefix, ncount, nchild, nfirst, geom, wn, ext,
+            noct, refit, err, [&](uint32_t s, const double* o) {
+                if (s < lim)
+                    return sp[s - b0];
+                return make_float4(static_cast<float>(px[s] - o[0]),
+                                   static_cast<float>(py[s] - o[1]),
+                                   static_cast<float>(pz[s] - o[2]), pm[s]);
+            });
+    }
 }
 
 // sleaf[s] = the leaf holding particle s (topology built level by level).
@@ -1263,12 +1307,12 @@ void launch_tree_moments(Device& d, bool refit)
     const unsigned gl = static_cast<unsigned>((d.n + kLeafChunk - 1) / kLeafChunk);
     if (gl > 0) {
         if (d.noct)
-            leaf_moments_kernel<true><<<gl, kLeafChunk, 0, d.stream>>>(
+            leaf_moments_kernel<true><<<gl, kLeafThreads, 0, d.stream>>>(
                 d.nprefix, d.nlevel, d.nstart, d.ncount, d.nchild, d.nfirst, d.sleaf, n,
                 d.level_info, d.pos[0], d.pos[1], d.pos[2], d.mass, d.geom, d.wnode, d.spart,
                 d.next, d.noct, rf, d.err);
         else
-            leaf_moments_kernel<false><<<gl, kLeafChunk, 0, d.stream>>>(
+            leaf_moments_kernel<false><<<gl, kLeafThreads, 0, d.stream>>>(
                 d.nprefix, d.nlevel, d.nstart, d.ncount, d.nchild, d.nfirst, d.sleaf, n,
                 d.level_info, d.pos[0], d.pos[1], d.pos[2], d.mass, d.geom, d.wnode, d.spart,
                 d.next, nullptr, rf, d.err);
