@@ -830,19 +830,32 @@ __device__ inline void cell_origin(const int* c, const RootGeom* __restrict__ ge
 // centre through part(s, origin) (particle s, as stored in spart); an internal
 // node combines its children, which must be complete. OCT: also the third
 // moments (noct, 10 per node; read by the learned far-field correction).
+struct NodeMeta {
+    unsigned long long prefix;
+    uint32_t nchild, first, count;
+};
+
+// The caller loads the node's fields (all in one round trip).
+__device__ inline NodeMeta node_meta(int node, const uint64_t* __restrict__ nprefix,
+                                     const uint32_t* __restrict__ ncount,
+                                     const uint32_t* __restrict__ nchild,
+                                     const uint32_t* __restrict__ nfirst)
+{
+    return NodeMeta{nprefix[node], nchild[node], nfirst[node], ncount[node]};
+}
+
 template <bool OCT, bool LEAF, class Part>
-__device__ inline void node_moments(
-    int node, int level, const uint64_t* __restrict__ nprefix, const uint32_t* __restrict__ ncount,
-    const uint32_t* __restrict__ nchild, const uint32_t* __restrict__ nfirst,
-    const RootGeom* __restrict__ geom, WalkNode* __restrict__ wn, float* __restrict__ ext,
-    float* __restrict__ noct, int refit, int* err, Part part)
+__device__ inline void node_moments(int node, int level, const NodeMeta& meta,
+                                    const RootGeom* __restrict__ geom, WalkNode* __restrict__ wn,
+                                    float* __restrict__ ext, float* __restrict__ noct, int refit,
+                                    int* err, Part part)
 {
     const float ux = geom->unit_f[0], uy = geom->unit_f[1], uz = geom->unit_f[2];
     {
         int c[3];
-        lattice_centre(nprefix[node], level, c);
-        const uint32_t nc = LEAF ? 0u : nchild[node];
-        const uint32_t first = nfirst[node];
+        lattice_centre(meta.prefix, level, c);
+        const uint32_t nc = LEAF ? 0u : meta.nchild;
+        const uint32_t first = meta.first;
         float M = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
         float qxx = 0.0f, qxy = 0.0f, qxz = 0.0f, qyy = 0.0f, qyz = 0.0f, bmax = 0.0f, trs = 0.0f;
         // e (refit only): half-width, in lattice units along the widest
@@ -855,7 +868,7 @@ __device__ inline void node_moments(
         if (OCT)
             oct_zero(oc);
         if (LEAF) {
-            count = ncount[node];
+            count = meta.count;
             double o[3];
             cell_origin(c, geom, o);
             float sx = 0.0f, sy = 0.0f, sz = 0.0f;
@@ -988,7 +1001,7 @@ __device__ inline void node_moments(
         if (nc != 0) {
             // internal node: the subtree's particle count, saturated (the walk
             // reads it only to pick the near-field treatment)
-            count = min(ncount[node], kMaxLeafCount);
+            count = min(meta.count, kMaxLeafCount);
         } else if (count > kMaxLeafCount) {
             // The walk then sees a truncated leaf: flag it; every particle was
             // still packed above, so no access goes out of bounds.
@@ -1067,9 +1080,10 @@ __global__ __launch_bounds__(kLeafThreads) void leaf_moments_kernel(
     for (uint32_t t = threadIdx.x; t < ns; t += kLeafThreads) {
         const uint32_t p = b0 + starts[t];
         const uint32_t leaf = sleaf[p];
+        const NodeMeta meta = node_meta(static_cast<int>(leaf), nprefix, ncount, nchild, nfirst);
         node_moments<OCT, true>(
-            static_cast<int>(leaf), nlevel[leaf], nprefix, ncount, nchild, nfirst, geom, wn, ext,
-            noct, refit, err, [&](uint32_t s, const double* o) {
+            static_cast<int>(leaf), nlevel[leaf], meta, geom, wn, ext, noct, refit, err,
+            [&](uint32_t s, const double* o) {
                 if (s < lim)
                     return sp[s - b0];
                 return make_float4(static_cast<float>(px[s] - o[0]),
@@ -1096,7 +1110,10 @@ __global__ __launch_bounds__(256) void leaf_fill_kernel(const uint32_t* __restri
     }
 }
 
-// Internal nodes of one level (deepest level first, after the leaves).
+// Internal nodes of one level (deepest level first, after the leaves), one
+// thread per node of the level. (Listing the internal ones first so that
+// whole wavefronts sum them was slower: the level is latency-bound, and the
+// mostly idle wavefronts of this form keep more loads in flight.)
 template <bool OCT>
 __global__ __launch_bounds__(256) void tree_moments_kernel(
     const uint64_t* __restrict__ nprefix, const uint32_t* __restrict__ ncount,
@@ -1109,10 +1126,11 @@ __global__ __launch_bounds__(256) void tree_moments_kernel(
     const int cnt = info[kLevels + level];
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
         const int node = base + k;
-        if (nchild[node] == 0)
+        const NodeMeta meta = node_meta(node, nprefix, ncount, nchild, nfirst);
+        if (meta.nchild == 0)
             continue;
-        node_moments<OCT, false>(node, level, nprefix, ncount, nchild, nfirst, geom, wn, ext, noct,
-                                 refit, err, [](uint32_t, const double*) { return float4{}; });
+        node_moments<OCT, false>(node, level, meta, geom, wn, ext, noct, refit, err,
+                                 [](uint32_t, const double*) { return float4{}; });
     }
 }
 
@@ -1320,12 +1338,13 @@ void launch_tree_moments(Device& d, bool refit)
     // level kMaxLevel holds leaves only
     for (int L = kMaxLevel - 1; L >= 0; --L) {
         const size_t bound = level_bound(L, d.n, d.node_cap);
+        const unsigned g = level_grid(bound);
         if (d.noct)
-            tree_moments_kernel<true><<<level_grid(bound), 256, 0, d.stream>>>(
+            tree_moments_kernel<true><<<g, 256, 0, d.stream>>>(
                 d.nprefix, d.ncount, d.nchild, d.nfirst, d.level_info, L, d.geom, d.wnode, d.next,
                 d.noct, rf, d.err);
         else
-            tree_moments_kernel<false><<<level_grid(bound), 256, 0, d.stream>>>(
+            tree_moments_kernel<false><<<g, 256, 0, d.stream>>>(
                 d.nprefix, d.ncount, d.nchild, d.nfirst, d.level_info, L, d.geom, d.wnode, d.next,
                 nullptr, rf, d.err);
     }
