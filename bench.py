@@ -14,16 +14,18 @@ initial conditions (generated on the host, uploaded once).
 
 The opening test is the accuracy knob of the metric. The headline runs the
 relative criterion: a node is accepted when its estimated truncation error,
-M bmax^3 / (d^2 + eps^2)^(5/2), is below alpha = 0.015 times the group's
+M bmax^3 / (d^2 + eps^2)^(5/2), is below alpha = 0.0125 times the group's
 smallest |a| of the previous force evaluation, with the opening angle bounded
 by theta = 2; accepted nodes whose quadrupole term, M bmax^2 / (d^2 +
-eps^2)^2, is below 0.1 alpha |a| enter as point masses. On this state that
-holds the median error near 6.6e-4, a third below the bar. The same run also times the plain COM walk at
-theta = 0.5 under "theta_0.5", the opening angle the workload is usually
-quoted at. Both force errors are measured in the run. `--criterion com
---theta 0.75 --far-alpha 0.75` runs the geometric criterion with the group
-far-field expansion instead; `--lff` adds the learned far-field correction
-(theta 0.9, trained on a second realisation, seed 12).
+eps^2)^2, is below 0.2 alpha |a| enter as point masses. On this state that
+holds the median error near 6.6e-4, a third below the bar. The same run also
+times the plain COM walk at theta = 0.5 under "theta_0.5", the opening angle
+the workload is usually quoted at. Both force errors are measured in the run.
+`--criterion com --theta 0.75 --far-alpha 0.75` runs the geometric criterion
+with the group far-field expansion instead. The run also times the block
+time-step, periodic TreePM and learned-correction workloads (theta 0.9,
+trained on a second realisation, seed 12); `--block-steps 0`, `--treepm 0`
+and `--no-lff` skip them.
 
 Prints one JSON line (see the keys below). `--gpus N` with N > 1 exits: this
 build runs on one GPU.
@@ -108,7 +110,7 @@ def cpu_baseline(pos, m, theta, eps, n_targets, seed=123):
     }
 
 
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "walk_build_merger8M_relative0.015_mono0.1_pmc.md")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "walk_build_merger8M_relative0.0125_mono0.2_pmc.md")
 
 
 def walk_traffic(args, n):
@@ -118,7 +120,7 @@ def walk_traffic(args, n):
     1 GiB copy in the same run measures them), or None for other workloads."""
     if not (args.workload == "merger" and n == 8388608 and args.order == 2 and args.bucket == 16
             and args.criterion == "relative" and args.theta == 2.0
-            and args.relative_alpha == 0.015 and args.relative_mono == 0.1
+            and args.relative_alpha == 0.0125 and args.relative_mono == 0.2
             and args.far_alpha == 0.0):
         return None
     try:
@@ -396,8 +398,8 @@ def main(argv=None):
     ap.add_argument("--criterion", default="relative", choices=["relative", "com", "bounds", "extent"])
     ap.add_argument("--theta", type=float, default=2.0,
                     help="opening angle (under the relative criterion: its upper bound)")
-    ap.add_argument("--relative-alpha", type=float, default=0.015)
-    ap.add_argument("--relative-mono", type=float, default=0.1,
+    ap.add_argument("--relative-alpha", type=float, default=0.0125)
+    ap.add_argument("--relative-mono", type=float, default=0.2,
                     help="relative criterion: point-mass tolerance as a fraction of alpha (0 = off)")
     ap.add_argument("--far-alpha", type=float, default=0.0,
                     help="group far-field expansion (0 = off)")

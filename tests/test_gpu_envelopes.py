@@ -236,8 +236,8 @@ def test_bitwise_replay():
                    theta=0.35, softening=0.08, integrator="euler")
     say("case replay plummer n=200000 relative criterion 5 kdk steps")
     assert _replay(200000, lambda: B.plummer(200000, seed=3), 5, 1e-3, theta=2.0,
-                   softening=0.01, criterion="relative", relative_alpha=0.015,
-                   relative_mono=0.1, integrator="leapfrog")
+                   softening=0.01, criterion="relative", relative_alpha=0.0125,
+                   relative_mono=0.2, integrator="leapfrog")
     say("case replay hernquist n=100000 block time steps")
     assert _replay(100000, lambda: B.hernquist(100000, seed=4), 2, 2.0 ** -6, theta=0.5,
                    softening=0.01, adaptive_max_level=6, adaptive_eta=0.02, adaptive_flags=0)
