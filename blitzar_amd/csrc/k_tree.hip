@@ -398,11 +398,15 @@ __device__ inline uint32_t wave_total(uint32_t v)
 }
 
 // Per tile: the level range of every particle (pinfo) and the nodes per
-// level (tilecnt[L * ntiles + tile]).
+// level (tilecnt[L * ntiles + tile]). BT > 0: the bucket as a compile-time
+// constant (the window loops unroll and their LDS reads overlap; one read at
+// a time they set the kernel's time); BT = 0: the bucket Brt at run time.
+template <int BT>
 __global__ __launch_bounds__(kTopoThreads) void topo_count_kernel(
-    const unsigned long long* __restrict__ keys, uint32_t n, int B, int D,
+    const unsigned long long* __restrict__ keys, uint32_t n, int Brt, int D,
     uint32_t* __restrict__ pinfo, uint32_t* __restrict__ tilecnt, uint32_t ntiles)
 {
+    const int B = BT > 0 ? BT : Brt;
     __shared__ TopoTile S;
     const int tid = threadIdx.x;
     const uint32_t tile = blockIdx.x;
@@ -432,8 +436,10 @@ __global__ __launch_bounds__(kTopoThreads) void topo_count_kernel(
     __syncthreads();
     for (int m = tid; m < kTopoTile + kTopoHalo; m += kTopoThreads) {
         int v = kMaxLevel;
-        for (int b = 1; b <= B; ++b)  // m + B < kTopoTile + 2 * kTopoHalo
-            v = min(v, static_cast<int>(S.d[m + b]));
+#pragma unroll
+        for (int b = 1; b <= (BT > 0 ? BT : kTopoHalo); ++b)  // m + B < kTopoTile + 2 * kTopoHalo
+            if (BT > 0 || b <= B)
+                v = min(v, static_cast<int>(S.d[m + b]));
         S.w[m] = static_cast<int8_t>(v);
     }
     __syncthreads();
@@ -446,16 +452,31 @@ __global__ __launch_bounds__(kTopoThreads) void topo_count_kernel(
         int lo = kLevels, hi = -1, c = 0;
         if (i < n) {
             int W = -1;
-            for (int b = 0; b <= B; ++b)  // r + H - b >= 0
-                W = max(W, static_cast<int>(S.w[r + kTopoHalo - b]));
+            if (BT > 0) {
+#pragma unroll
+                for (int b = 0; b <= BT; ++b)  // r + H - b >= 0
+                    W = max(W, static_cast<int>(S.w[r + kTopoHalo - b]));
+            } else {
+                for (int b = 0; b <= B; ++b)
+                    W = max(W, static_cast<int>(S.w[r + kTopoHalo - b]));
+            }
             lo = i == 0 ? 0 : static_cast<int>(S.d[r + kTopoHalo]) + 1;
             hi = min(kMaxLevel, max(D, W + 1));
             // the leaf at hi ends at the next run start within B, if any
-            // (d is -1 at and beyond n)
-            for (int b = 1; b <= B && lo <= hi; ++b) {
-                if (S.d[r + kTopoHalo + b] < hi) {
-                    c = b;
-                    break;
+            // (d is -1 at and beyond n): the smallest such b
+            if (lo <= hi) {
+                if (BT > 0) {
+#pragma unroll
+                    for (int b = BT; b >= 1; --b)
+                        if (S.d[r + kTopoHalo + b] < hi)
+                            c = b;
+                } else {
+                    for (int b = 1; b <= B; ++b) {
+                        if (S.d[r + kTopoHalo + b] < hi) {
+                            c = b;
+                            break;
+                        }
+                    }
                 }
             }
         }
@@ -1268,8 +1289,12 @@ void launch_tree_build(Device& d, int leaf_depth, int bucket)
     if (n > 0 && bucket <= kTopoHalo && d.topo_cnt) {
         const uint32_t ntiles = topo_tiles(d.n);
         uint32_t* pinfo = d.bnd_tmp;  // n words (the level-by-level build's child bounds)
-        topo_count_kernel<<<ntiles, kTopoThreads, 0, d.stream>>>(sorted, n, bucket, leaf_depth,
-                                                                 pinfo, d.topo_cnt, ntiles);
+        if (bucket == 16)  // the default
+            topo_count_kernel<16><<<ntiles, kTopoThreads, 0, d.stream>>>(
+                sorted, n, bucket, leaf_depth, pinfo, d.topo_cnt, ntiles);
+        else
+            topo_count_kernel<0><<<ntiles, kTopoThreads, 0, d.stream>>>(
+                sorted, n, bucket, leaf_depth, pinfo, d.topo_cnt, ntiles);
         topo_scan_kernel<<<kLevels, kScanBlock, 0, d.stream>>>(d.topo_cnt, ntiles);
         topo_emit_kernel<<<ntiles, kTopoThreads, 0, d.stream>>>(
             sorted, n, pinfo, d.topo_cnt, ntiles, static_cast<int>(d.node_cap), d.nprefix,

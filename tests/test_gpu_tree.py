@@ -77,6 +77,28 @@ def test_topology_bit_exact(oracle, n, leaf_cap, bucket):
         assert np.array_equal(t[key].astype(np.int64), ref[key].astype(np.int64)), key
 
 
+def test_topology_bit_exact_hernquist_merger_8388608(oracle):
+    """The benchmark state: 8,388,608 particles in two Hernquist halos (2,048
+    topology tiles, about 2.1 million nodes down to level 17), the whole tree
+    against the oracle."""
+    import blitzar_amd as B
+    n = 8388608
+    say(f"case topology hernquist merger n={n}")
+    pos, vel, m = B.hernquist_merger(n // 2, seed=11, separation=20.0, pericentre=2.0)
+    with _solver(n, leaf_capacity=16, bucket_capacity=16) as s:
+        s.upload(pos, vel, m)
+        s.build_tree()
+        t = s.dump_tree()
+    sk, order = oracle.stable_sort(oracle.keys(pos, 0))
+    assert np.array_equal(t["sorted_keys"], sk)
+    assert np.array_equal(t["sorted_ids"], order)
+    ref = oracle.hip_tree(sk, oracle.leaf_depth(n, 16), 16)
+    say(f"  nodes {len(ref['level'])}, deepest level {int(ref['level'].max())}")
+    assert len(t["level"]) == len(ref["level"])
+    for key in ("level", "prefix", "start", "count", "first_child", "nchild"):
+        assert np.array_equal(t[key].astype(np.int64), ref[key].astype(np.int64)), key
+
+
 @pytest.mark.parametrize("key_mode", [0, 1])
 def test_reference_tree_is_truncation(oracle, key_mode):
     n = 30000
