@@ -390,6 +390,32 @@ __device__ inline int topo_lo(uint32_t v) { return static_cast<int>(v & 0xffu); 
 __device__ inline int topo_hi(uint32_t v) { return static_cast<int>((v >> 8) & 0xffu) - 1; }
 __device__ inline int topo_c(uint32_t v) { return static_cast<int>((v >> 16) & 0xffu); }
 
+// A thread's node count per level, in registers: 5-bit fields (at most 16
+// particles per thread), levels 0-10 in c[0] and 11-21 in c[1].
+struct LevelCounts {
+    unsigned long long c[2] = {0ull, 0ull};
+
+    // fields a..b (0 <= a <= b <= 10) of one word set to 1
+    __device__ static unsigned long long fields(int a, int b)
+    {
+        constexpr unsigned long long kOnes = 0x4210842108421ull;  // bit 5k, k = 0..10
+        return kOnes & ((1ull << (5 * (b + 1))) - 1ull) & ~((1ull << (5 * a)) - 1ull);
+    }
+    __device__ void add(int lo, int hi)
+    {
+        if (lo > hi)
+            return;
+        if (lo <= 10)
+            c[0] += fields(lo, min(hi, 10));
+        if (hi >= 11)
+            c[1] += fields(max(lo, 11) - 11, hi - 11);
+    }
+    __device__ uint32_t get(int L) const
+    {
+        return static_cast<uint32_t>((L <= 10 ? c[0] >> (5 * L) : c[1] >> (5 * (L - 11))) & 31u);
+    }
+};
+
 __device__ inline uint32_t wave_total(uint32_t v)
 {
     for (int o = 32; o > 0; o >>= 1)
@@ -431,8 +457,6 @@ __global__ __launch_bounds__(kTopoThreads) void topo_count_kernel(
                                              ? shared_digits(ka[k], kb[k])
                                              : -1);
     }
-    for (int L = 0; L < kLevels; ++L)
-        S.cnt[L][tid] = 0u;
     __syncthreads();
     for (int m = tid; m < kTopoTile + kTopoHalo; m += kTopoThreads) {
         int v = kMaxLevel;
@@ -445,6 +469,7 @@ __global__ __launch_bounds__(kTopoThreads) void topo_count_kernel(
     __syncthreads();
     // particles interleaved over the threads (neighbouring lanes read
     // neighbouring bytes of d and w, and store neighbouring words)
+    LevelCounts lc;
 #pragma unroll 4
     for (int q = 0; q < kTopoPer; ++q) {
         const int r = q * kTopoThreads + tid;
@@ -482,9 +507,11 @@ __global__ __launch_bounds__(kTopoThreads) void topo_count_kernel(
         }
         if (i < n)
             pinfo[i] = topo_pack(lo, hi, c);
-        for (int L = lo; L <= hi; ++L)
-            S.cnt[L][tid] += 1u;
+        lc.add(lo, hi);
     }
+#pragma unroll
+    for (int L = 0; L < kLevels; ++L)
+        S.cnt[L][tid] = lc.get(L);
     __syncthreads();
     const int lane = tid & 63, wv = tid >> 6;
     for (int L = wv; L < kLevels; L += kTopoThreads / 64) {
@@ -612,12 +639,13 @@ __global__ __launch_bounds__(kTopoThreads) void topo_emit_kernel(
             lmax_t = max(lmax_t, min(hi, lmax));
         }
     }
-    for (int L = 0; L < kLevels; ++L)
-        cnt[L][tid] = 0u;
+    LevelCounts lc;
 #pragma unroll
     for (int q = 0; q < kTopoPer; ++q)
-        for (int L = topo_lo(pk[q]); L <= topo_hi(pk[q]); ++L)
-            cnt[L][tid] += 1u;
+        lc.add(topo_lo(pk[q]), topo_hi(pk[q]));
+#pragma unroll
+    for (int L = 0; L < kLevels; ++L)
+        cnt[L][tid] = lc.get(L);
     __syncthreads();
     // per level: exclusive scan of the thread counts, in place
     const int lane = tid & 63, wv = tid >> 6;
@@ -705,9 +733,18 @@ __device__ inline uint32_t topo_children(const uint64_t* __restrict__ nprefix,
 {
     const uint32_t end =
         static_cast<uint32_t>(info[L + 1]) + static_cast<uint32_t>(info[kLevels + L + 1]);
+    // the next 7 slots' prefixes, loaded together (in bounds: clamped to f)
+    unsigned long long q[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+        q[k] = nprefix[f + k + 1u < end ? f + k + 1u : f];
     uint32_t m = 1;
-    while (m < 8u && f + m < end && (nprefix[f + m] >> 3) == p)
-        ++m;
+    bool run = true;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        run = run && f + k + 1u < end && (q[k] >> 3) == p;
+        m += run ? 1u : 0u;
+    }
     return m;
 }
 
@@ -729,9 +766,18 @@ __global__ __launch_bounds__(256) void topo_link_kernel(
             // a later sibling: its parent is that of the first sibling (the
             // parent's first child, linked by the emit pass), at most 7 back
             const int lb = info[L];
+            // the 7 slots before, loaded together (in bounds: clamped to x)
+            unsigned long long q[7];
+#pragma unroll
+            for (int k = 0; k < 7; ++k)
+                q[k] = nprefix[x - k - 1 >= lb ? x - k - 1 : x];
             int y = x;
-            for (int k = 0; k < 7 && y - 1 >= lb && (nprefix[y - 1] >> 3) == (px >> 3); ++k)
-                --y;
+            bool run = true;
+#pragma unroll
+            for (int k = 0; k < 7; ++k) {
+                run = run && x - k - 1 >= lb && (q[k] >> 3) == (px >> 3);
+                y -= run ? 1 : 0;
+            }
             const uint32_t par = nparent[y];
             if (y == x || par == kPending)
                 atomicOr(err, kErrTopology);
