@@ -30,7 +30,7 @@ constexpr size_t kStackCap = 16384;
 // moments and groups), w1 walk end.
 struct StepEvents {
     hipEvent_t b0 = nullptr, s0 = nullptr, s1 = nullptr, p1 = nullptr, t1 = nullptr,
-               b1 = nullptr, w1 = nullptr;
+               j1 = nullptr, b1 = nullptr, w1 = nullptr;
     hipEvent_t l0 = nullptr;  // learned correction start (after its walk)
     bool l0_used = false;
 };
@@ -66,6 +66,11 @@ struct bh_hip_handle {
     // that marks the copy; read while the walk queued behind the build runs.
     int* err_host = nullptr;
     hipEvent_t built = nullptr;
+    // The topology and the walk groups of a build run on a second stream
+    // (default priority), beside the store permutation and the moments:
+    // sorted -> topology (side), topo -> moments (main), groups -> walk (main).
+    hipStream_t side = nullptr;
+    hipEvent_t ev_sorted = nullptr, ev_topo = nullptr, ev_groups = nullptr;
     int err_carry = 0;      // error bits of this call kept across a node regrow
     int steps_done = -1;    // bh_hip_step: steps completed before an error stopped it
     // Test switch (environment BH_HIP_TEST_FAIL_NODE_GROWTH at creation): a
@@ -398,7 +403,7 @@ StepEvents* next_events(bh_hip_handle* h)
         return nullptr;
     if (h->events_used == h->events.size()) {
         StepEvents e;
-        for (hipEvent_t* x : {&e.b0, &e.s0, &e.s1, &e.p1, &e.t1, &e.b1, &e.w1, &e.l0})
+        for (hipEvent_t* x : {&e.b0, &e.s0, &e.s1, &e.p1, &e.t1, &e.j1, &e.b1, &e.w1, &e.l0})
             if (hipEventCreate(x) != hipSuccess)
                 return nullptr;
         h->events.push_back(e);
@@ -442,15 +447,25 @@ void queue_build(bh_hip_handle* h, StepEvents* ev)
     bh::launch_radix_sort(d, d.n, h->p.key_mode == 1 ? 30 : 63);
     rec(ev ? ev->s1 : nullptr, d.stream);
     bh::launch_tie_fix(d);
+    // The topology needs only the sorted keys: it runs on the side stream
+    // while the main stream permutes the particle store. The walk groups
+    // follow it there, beside the moments.
+    (void)hipEventRecord(h->ev_sorted, d.stream);
+    (void)hipStreamWaitEvent(h->side, h->ev_sorted, 0);
+    bh::launch_tree_build(d, h->leaf_depth, h->p.bucket_capacity, h->side);
+    rec(ev ? ev->t1 : nullptr, h->side);
+    (void)hipEventRecord(h->ev_topo, h->side);
+    if (h->p.key_mode == 0)
+        bh::launch_groups(d, h->side);
+    (void)hipEventRecord(h->ev_groups, h->side);
     bh::launch_permute(d);
     rec(ev ? ev->p1 : nullptr, d.stream);
     h->rank_valid = false;
-    bh::launch_tree_build(d, h->leaf_depth, h->p.bucket_capacity);
-    rec(ev ? ev->t1 : nullptr, d.stream);
-    if (h->p.key_mode == 0) {
+    (void)hipStreamWaitEvent(d.stream, h->ev_topo, 0);
+    rec(ev ? ev->j1 : nullptr, d.stream);
+    if (h->p.key_mode == 0)
         bh::launch_tree_moments(d);
-        bh::launch_groups(d);
-    }
+    (void)hipStreamWaitEvent(d.stream, h->ev_groups, 0);
     rec(ev ? ev->b1 : nullptr, d.stream);
     (void)hipMemcpyAsync(h->err_host, d.err, sizeof(int), hipMemcpyDeviceToHost, d.stream);
     (void)hipEventRecord(h->built, d.stream);
@@ -463,7 +478,7 @@ void queue_refit(bh_hip_handle* h, StepEvents* ev)
 {
     bh::Device& d = h->d;
     for (hipEvent_t e : {ev ? ev->b0 : nullptr, ev ? ev->s0 : nullptr, ev ? ev->s1 : nullptr,
-                         ev ? ev->p1 : nullptr, ev ? ev->t1 : nullptr})
+                         ev ? ev->p1 : nullptr, ev ? ev->t1 : nullptr, ev ? ev->j1 : nullptr})
         rec(e, d.stream);
     bh::launch_tree_moments(d, true);
     rec(ev ? ev->b1 : nullptr, d.stream);
@@ -632,8 +647,8 @@ void end_call(bh_hip_handle* h)
             span(e.b0, e.s0, t.keys_ms);
             span(e.s0, e.s1, t.sort_ms);
             span(e.s1, e.p1, t.permute_ms);
-            span(e.p1, e.t1, t.tree_ms);
-            span(e.t1, e.b1, t.moments_ms);
+            span(e.s1, e.t1, t.tree_ms);
+            span(e.j1, e.b1, t.moments_ms);
             span(e.b1, e.w1, t.walk_ms);
             if (e.l0_used)
                 span(e.l0, e.w1, t.lff_ms);
@@ -908,7 +923,10 @@ bh_hip_result_t bh_hip_create(const bh_hip_params_t* params, uint64_t n, bh_hip_
         prop.multiProcessorCount > 0)
         h->cu_count = prop.multiProcessorCount;
     h->leaf_depth = reference_leaf_depth(n, h->p.leaf_capacity);
-    bool ok = check(h, hipStreamCreateWithFlags(&h->d.stream, hipStreamNonBlocking), "stream");
+    bool ok = check(h, hipStreamCreateWithFlags(&h->d.stream, hipStreamNonBlocking), "stream") &&
+              check(h, hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking), "stream");
+    for (hipEvent_t* x : {&h->ev_sorted, &h->ev_topo, &h->ev_groups})
+        ok = ok && check(h, hipEventCreateWithFlags(x, hipEventDisableTiming), "event");
     ok = ok && alloc_all(h, n);
     ok = ok && check(h, hipEventCreateWithFlags(&h->built, hipEventDisableTiming), "event") &&
          check(h, hipHostMalloc(reinterpret_cast<void**>(&h->err_host), sizeof(int),
@@ -939,12 +957,14 @@ void bh_hip_destroy(bh_hip_handle_t* h)
         return;
     if (h->d.stream)
         (void)hipStreamSynchronize(h->d.stream);
+    if (h->side)
+        (void)hipStreamSynchronize(h->side);
     bh::pm_release(h->pm);
     for (void* p : h->allocations)
         (void)hipFree(p);
     h->allocations.clear();
     for (auto& e : h->events) {
-        for (hipEvent_t x : {e.b0, e.s0, e.s1, e.p1, e.t1, e.b1, e.w1, e.l0})
+        for (hipEvent_t x : {e.b0, e.s0, e.s1, e.p1, e.t1, e.j1, e.b1, e.w1, e.l0})
             if (x)
                 (void)hipEventDestroy(x);
     }
@@ -954,10 +974,15 @@ void bh_hip_destroy(bh_hip_handle_t* h)
     }
     if (h->built)
         (void)hipEventDestroy(h->built);
+    for (hipEvent_t x : {h->ev_sorted, h->ev_topo, h->ev_groups})
+        if (x)
+            (void)hipEventDestroy(x);
     if (h->err_host)
         (void)hipHostFree(h->err_host);
     if (h->d.stream)
         (void)hipStreamDestroy(h->d.stream);
+    if (h->side)
+        (void)hipStreamDestroy(h->side);
     delete h;
 }
 

@@ -307,7 +307,7 @@ void launch_tie_fix(Device& d);
 void launch_permute(Device& d);
 void launch_rank(Device& d);  // rebuild rank from id
 // k_tree.hip
-void launch_tree_build(Device& d, int leaf_depth, int bucket);
+void launch_tree_build(Device& d, int leaf_depth, int bucket, hipStream_t s);
 // Moments, leaf packing and the walk's node records of the current
 // positions. refit: the topology is that of an earlier build (positions have
 // moved since); each node's level in the walk record is lowered until its
@@ -316,7 +316,7 @@ void launch_tree_moments(Device& d, bool refit = false);
 // Walk groups: a partition of the sorted particles into spatially compact
 // runs of at most kGroupMax (consecutive small children of one node packed
 // together; see node_groups in k_tree.hip).
-void launch_groups(Device& d);
+void launch_groups(Device& d, hipStream_t s);
 // k_walk.hip
 // stats (optional, device): [0] particle-particle, [1] particle-cell interactions,
 // [3] node-group far-field interactions, [4] far-field target evaluations

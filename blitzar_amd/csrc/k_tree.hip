@@ -1328,7 +1328,7 @@ inline size_t level_bound(int level, size_t n, size_t cap)
 
 }  // namespace
 
-void launch_tree_build(Device& d, int leaf_depth, int bucket)
+void launch_tree_build(Device& d, int leaf_depth, int bucket, hipStream_t s)
 {
     const uint32_t n = static_cast<uint32_t>(d.n);
     const unsigned long long* sorted = reinterpret_cast<const unsigned long long*>(d.keys);
@@ -1336,56 +1336,56 @@ void launch_tree_build(Device& d, int leaf_depth, int bucket)
         const uint32_t ntiles = topo_tiles(d.n);
         uint32_t* pinfo = d.bnd_tmp;  // n words (the level-by-level build's child bounds)
         if (bucket == 16)  // the default
-            topo_count_kernel<16><<<ntiles, kTopoThreads, 0, d.stream>>>(
+            topo_count_kernel<16><<<ntiles, kTopoThreads, 0, s>>>(
                 sorted, n, bucket, leaf_depth, pinfo, d.topo_cnt, ntiles);
         else
-            topo_count_kernel<0><<<ntiles, kTopoThreads, 0, d.stream>>>(
+            topo_count_kernel<0><<<ntiles, kTopoThreads, 0, s>>>(
                 sorted, n, bucket, leaf_depth, pinfo, d.topo_cnt, ntiles);
-        topo_scan_kernel<<<kLevels, kScanBlock, 0, d.stream>>>(d.topo_cnt, ntiles);
-        topo_emit_kernel<<<ntiles, kTopoThreads, 0, d.stream>>>(
+        topo_scan_kernel<<<kLevels, kScanBlock, 0, s>>>(d.topo_cnt, ntiles);
+        topo_emit_kernel<<<ntiles, kTopoThreads, 0, s>>>(
             sorted, n, pinfo, d.topo_cnt, ntiles, static_cast<int>(d.node_cap), d.nprefix,
             d.nlevel, d.nstart, d.ncount, d.nchild, d.nfirst, d.nparent, d.level_info, d.err);
         topo_link_kernel<<<level_grid(std::min<size_t>(d.node_cap, 2 * d.n + 64)), 256, 0,
-                           d.stream>>>(sorted, n, d.nprefix, d.nlevel, d.nstart, d.ncount,
+                           s>>>(sorted, n, d.nprefix, d.nlevel, d.nstart, d.ncount,
                                        d.nchild, d.nfirst, d.nparent, d.sleaf, d.level_info,
                                        d.err);
         return;
     }
     // Buckets wider than the tiles' halo: level by level, top down.
-    tree_root_kernel<<<1, 64, 0, d.stream>>>(d.nprefix, d.nlevel, d.nstart, d.ncount, d.nchild,
+    tree_root_kernel<<<1, 64, 0, s>>>(d.nprefix, d.nlevel, d.nstart, d.ncount, d.nchild,
                                              d.nfirst, d.nparent, d.level_info, n);
     const unsigned long long* keys = reinterpret_cast<const unsigned long long*>(d.keys);
     const int cap = static_cast<int>(d.node_cap);
     for (int L = 0; L < kMaxLevel; ++L) {
         const size_t bound = level_bound(L, d.n, d.node_cap);
         const unsigned g = level_grid(bound * 64);  // a wavefront per node on small levels
-        tree_count_kernel<<<g, 256, 0, d.stream>>>(
+        tree_count_kernel<<<g, 256, 0, s>>>(
             keys, d.nstart, d.ncount, d.nchild, d.nfirst, d.scan_tmp, d.bnd_tmp, d.big_list,
             d.level_info, L, leaf_depth, static_cast<uint32_t>(bucket));
         if (bound > kWavePerNodeMax)  // levels that can run thread-per-node
-            tree_count_big_kernel<<<64, 256, 0, d.stream>>>(keys, d.nstart, d.ncount, d.nchild,
+            tree_count_big_kernel<<<64, 256, 0, s>>>(keys, d.nstart, d.ncount, d.nchild,
                                                             d.nfirst, d.scan_tmp, d.bnd_tmp,
                                                             d.big_list, d.level_info, L);
         const unsigned nscan =
             static_cast<unsigned>(std::max<size_t>(1, (bound + kScanChunk - 1) / kScanChunk));
-        tree_scan_fix_kernel<<<nscan, kScanBlock, 0, d.stream>>>(d.scan_tmp, d.scan_blocks,
+        tree_scan_fix_kernel<<<nscan, kScanBlock, 0, s>>>(d.scan_tmp, d.scan_blocks,
                                                                  d.level_info, L, cap, d.err);
-        tree_emit_kernel<<<level_grid(bound), 256, 0, d.stream>>>(
+        tree_emit_kernel<<<level_grid(bound), 256, 0, s>>>(
             d.nprefix, d.nlevel, d.nstart, d.ncount, d.nchild, d.nfirst, d.nparent, d.scan_tmp,
             d.bnd_tmp, d.level_info, L);
     }
     leaf_fill_kernel<<<level_grid(std::min<size_t>(d.node_cap, 2 * d.n + 64)), 256, 0,
-                       d.stream>>>(d.nstart, d.ncount, d.nchild, d.level_info, n, d.sleaf);
+                       s>>>(d.nstart, d.ncount, d.nchild, d.level_info, n, d.sleaf);
 }
 
-void launch_groups(Device& d)
+void launch_groups(Device& d, hipStream_t s)
 {
     const unsigned g = level_grid(d.node_cap);
-    group_count_kernel<<<g, 256, 0, d.stream>>>(d.nprefix, d.nstart, d.ncount, d.nchild, d.nfirst,
+    group_count_kernel<<<g, 256, 0, s>>>(d.nprefix, d.nstart, d.ncount, d.nchild, d.nfirst,
                                                 d.nparent, d.level_info, d.gscan);
     launch_scan_exclusive(d.gscan, d.node_cap, d.level_info + kInfoTotal, 0, d.gscan_blocks,
-                          nullptr, d.level_info + kInfoGroups, d.err, d.stream);
-    group_emit_kernel<<<g, 256, 0, d.stream>>>(d.nprefix, d.nstart, d.ncount, d.nchild, d.nfirst,
+                          nullptr, d.level_info + kInfoGroups, d.err, s);
+    group_emit_kernel<<<g, 256, 0, s>>>(d.nprefix, d.nstart, d.ncount, d.nchild, d.nfirst,
                                                d.nparent, d.gscan, d.level_info, d.groups);
 }
 

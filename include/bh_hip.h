@@ -152,8 +152,10 @@ typedef struct bh_hip_timings {
     /* parts of build_ms */
     double keys_ms;     /* AABB + Morton keys */
     double permute_ms;  /* equal-key tie order + store permutation */
-    double tree_ms;     /* tree levels (topology) */
-    double moments_ms;  /* moments, leaf packing and walk groups */
+    double tree_ms;     /* topology, from the end of the sort (it runs beside the
+                           permutation, on a second stream) */
+    double moments_ms;  /* moments and leaf packing, after the topology, until the
+                           walk groups (second stream, beside them) are done */
     uint64_t lff_pairs; /* learned correction: (target, node) pairs evaluated */
     uint64_t lff_tiles; /* learned correction: 16-pair MFMA tiles run */
     double lff_ms;      /* learned correction: the MLP pass (part of walk_ms) */
