@@ -736,7 +736,12 @@ bool build_and_walk(bh_hip_handle* h, StepEvents* ev, const bh::WalkParams& wp,
             queue_refit(h, ev);
         else
             queue_build(h, ev);
-        launch_force_walk(h, wp);
+        // A build permutes the block time-step levels with the store (the
+        // buffer pointers swap): the walk must read the permuted ones.
+        bh::WalkParams w = wp;
+        if (w.alevel)
+            w.alevel = h->d.alevel;
+        launch_force_walk(h, w);
         const Built b = check_build(h);
         if (b == Built::kOk) {
             rec(ev ? ev->w1 : nullptr, h->d.stream);

@@ -479,8 +479,13 @@ __global__ __launch_bounds__(64, FAR ? 1 : BH_WALK_MIN_WAVES) void walk_kernel(
         const int gnode = static_cast<int>(gr.z);
         const int rcx = nodes[gnode].cx, rcy = nodes[gnode].cy, rcz = nodes[gnode].cz;
 
+        // Targets of this walk: the group's particles, or under block time
+        // steps those due a force now. The box, the relative tolerance and
+        // the stored forces cover these only (the corrector reads no other
+        // particle's force, and each particle is due again before its next
+        // kick), so a group with a few due particles walks with their box.
+        bool target = active;
         if (P.alevel) {
-            // block time steps: skip groups with no particle due a force now
             bool due = false;
             if (active) {
                 const uint32_t lv = min(P.alevel[s], static_cast<uint32_t>(P.alevels));
@@ -488,6 +493,7 @@ __global__ __launch_bounds__(64, FAR ? 1 : BH_WALK_MIN_WAVES) void walk_kernel(
             }
             if (!__any(due))
                 continue;
+            target = due;
         }
 
         float tx = 0.0f, ty = 0.0f, tz = 0.0f;
@@ -498,9 +504,9 @@ __global__ __launch_bounds__(64, FAR ? 1 : BH_WALK_MIN_WAVES) void walk_kernel(
             ty = rel_pos(nodes[lf].cy - rcy, uy, p.y);
             tz = rel_pos(nodes[lf].cz - rcz, uz, p.z);
         }
-        const float gx0 = wmin(active ? tx : 3.0e38f), gx1 = wmax(active ? tx : -3.0e38f);
-        const float gy0 = wmin(active ? ty : 3.0e38f), gy1 = wmax(active ? ty : -3.0e38f);
-        const float gz0 = wmin(active ? tz : 3.0e38f), gz1 = wmax(active ? tz : -3.0e38f);
+        const float gx0 = wmin(target ? tx : 3.0e38f), gx1 = wmax(target ? tx : -3.0e38f);
+        const float gy0 = wmin(target ? ty : 3.0e38f), gy1 = wmax(target ? ty : -3.0e38f);
+        const float gz0 = wmin(target ? tz : 3.0e38f), gz1 = wmax(target ? tz : -3.0e38f);
         const float gcx = 0.5f * (gx0 + gx1), gcy = 0.5f * (gy0 + gy1), gcz = 0.5f * (gz0 + gz1);
         const float ghx = 0.5f * (gx1 - gx0), ghy = 0.5f * (gy1 - gy0), ghz = 0.5f * (gz1 - gz0);
         // relative criterion: the group's smallest |a| of the last evaluation,
@@ -508,8 +514,8 @@ __global__ __launch_bounds__(64, FAR ? 1 : BH_WALK_MIN_WAVES) void walk_kernel(
         // has no meaningful relative error and would open everything)
         float rtol = 0.0f, rtol_mono = 0.0f;
         if (CRIT == 3) {
-            const float am = active ? P.amag[s] : 0.0f;
-            const float amin = wmin(active ? am : 3.0e38f), amax = wmax(am);
+            const float am = target ? P.amag[s] : 0.0f;
+            const float amin = wmin(target ? am : 3.0e38f), amax = wmax(am);
             rtol = P.ralpha * fmaxf(amin, 0.01f * amax);
             rtol_mono = P.rmono * rtol;
         }
@@ -819,7 +825,7 @@ __global__ __launch_bounds__(64, FAR ? 1 : BH_WALK_MIN_WAVES) void walk_kernel(
             rec_n[g] = static_cast<uint32_t>(nrec < rec_cap ? nrec : rec_cap);
         __syncthreads();
 
-        if (active && j0 == 0) {
+        if (target && j0 == 0) {
             // OctHostMath.cpp:50-57
             const float mag = sqrtf(ax * ax + ay * ay + az * az);
             if (P.max_acc > 0.0f && mag > P.max_acc && mag > 1.0e-6f) {
