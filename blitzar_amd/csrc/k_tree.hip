@@ -358,12 +358,13 @@ __global__ __launch_bounds__(256) void tree_emit_kernel(
 // the one at hi_s a leaf and each other the parent of the next. Level L's
 // nodes in breadth-first order are the particles with lo_s <= L <= hi_s in
 // increasing order, so a node's index is its level's base plus its rank
-// among them. Passes: per 4,096-particle tile, count nodes per level; scan
-// the counts per level; per tile, emit the nodes; then, per node, link what
-// a tile cannot see (parents of later siblings, child counts, counts of
-// internal nodes and of leaves longer than B).
+// among them. Passes: per 4,096-particle tile, each particle's [lo, hi] (one
+// word) and the tile's node count per level; a scan of the counts per level;
+// per tile, the nodes in place; then, per node, what a tile cannot see
+// (parents of later siblings, child counts, counts of internal nodes and of
+// leaves longer than B, each particle's leaf).
 constexpr int kTopoThreads = 256;
-constexpr int kTopoPer = kTopoTile / kTopoThreads;  // consecutive particles per thread
+constexpr int kTopoPer = kTopoTile / kTopoThreads;  // particles per thread (consecutive in the emit pass)
 constexpr uint32_t kPending = 0xfffffffeu;
 constexpr unsigned long long kKeyMask = 0x7fffffffffffffffull;
 
