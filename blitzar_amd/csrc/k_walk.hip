@@ -449,6 +449,7 @@ __global__ __launch_bounds__(64, FAR ? 1 : BH_WALK_MIN_WAVES) void walk_kernel(
     const float ux = geom->unit_f[0], uy = geom->unit_f[1], uz = geom->unit_f[2];
     const float wfx = geom->width_f[0], wfy = geom->width_f[1], wfz = geom->width_f[2];
     const float wfmax = fmaxf(fmaxf(wfx, wfy), wfz);
+    const float rcut = TPM && P.rcut2 > 0.0f ? sqrtf(P.rcut2) : 0.0f;
     const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     unsigned long long npp = 0, npc = 0;
     // this wave's interaction totals, added to `stats` once when it exits
@@ -629,7 +630,9 @@ __global__ __launch_bounds__(64, FAR ? 1 : BH_WALK_MIN_WAVES) void walk_kernel(
                         const float r2 = d2 + P.eps2;
                         const float b2 = w.bmax * w.bmax;
                         const float err = ORDER >= 2 ? w.mass * b2 * w.bmax : w.mass * b2;
-                        const float lim = ORDER >= 2 ? rtol * r2 * r2 * sqrtf(r2) : rtol * r2 * r2;
+                        // (v_sqrt_f32: the acceptance needs no correctly rounded root)
+                        const float lim = ORDER >= 2 ? rtol * r2 * r2 * __builtin_amdgcn_sqrtf(r2)
+                                                     : rtol * r2 * r2;
                         ok = ok && size * size < P.theta2 * r2 && err <= lim;
                         // quadrupole term within tolerance too: a point mass suffices
                         mono = ORDER >= 2 && ok && w.mass * b2 <= rtol_mono * r2 * r2;
@@ -639,7 +642,7 @@ __global__ __launch_bounds__(64, FAR ? 1 : BH_WALK_MIN_WAVES) void walk_kernel(
                     if (TPM && P.rcut2 > 0.0f && !wide) {
                         // Prune nodes whose bounding sphere lies beyond the cutoff for the
                         // whole group.
-                        const float lim = sqrtf(P.rcut2) + w.bmax;
+                        const float lim = rcut + w.bmax;
                         if (d2 > lim * lim) {
                             ok = false;
                             w.mass = 0.0f;
