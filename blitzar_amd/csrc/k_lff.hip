@@ -149,10 +149,12 @@ struct NodeFeat {
 __device__ inline NodeFeat node_feat(const WalkNode& w, const float* __restrict__ oct, float s,
                                      uint32_t cnt)
 {
+    // hardware reciprocal and log2 (v_rcp_f32, v_log_f32): the features need
+    // FP32 accuracy of a few ulps, not correct rounding
     NodeFeat n;
-    n.inv_s = 1.0f / s;
-    n.f[0] = log2f(static_cast<float>(cnt > 0 ? cnt : 1)) * (1.0f / 16.0f);
-    const float qn = w.mass > 0.0f ? 1.0f / (w.mass * s * s) : 0.0f;
+    n.inv_s = __builtin_amdgcn_rcpf(s);
+    n.f[0] = __builtin_amdgcn_logf(static_cast<float>(cnt > 0 ? cnt : 1)) * (1.0f / 16.0f);
+    const float qn = w.mass > 0.0f ? __builtin_amdgcn_rcpf(w.mass * s * s) : 0.0f;
     n.f[1] = w.qxx * qn;
     n.f[2] = w.qxy * qn;
     n.f[3] = w.qxz * qn;
@@ -772,7 +774,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void lf
                     }
                     const f32x4 c = mlp_mfma(R, sB, q, x0, x1);
                     if (q == 0 && valid) {
-                        const float mono = w.mass / (d2 + eps2);
+                        const float mono = w.mass * __builtin_amdgcn_rcpf(d2 + eps2);
                         sA[tgt][0] += c[0] * mono;
                         sA[tgt][1] += c[1] * mono;
                         sA[tgt][2] += c[2] * mono;
