@@ -83,27 +83,53 @@ __device__ inline int ticket_position(unsigned k, int q, int nq)
                              static_cast<unsigned>(q)) * kTicketChunk + k % kTicketChunk);
 }
 
-__device__ inline float wmin(float v)
+// Minimum / maximum over the 64 lanes (all active), returned to every lane:
+// rotations by 1, 2, 4 and 8 within each 16-lane row give every lane its
+// row's extremum, the row broadcasts carry rows 0-2 into row 3, and lane 63
+// is read back as a scalar. Lanes outside a broadcast's row mask combine
+// with themselves.
+template <bool MAX, int CTRL, int ROWS>
+__device__ inline float wextreme_step(float v)
 {
-    for (int o = 32; o > 0; o >>= 1)
-        v = fminf(v, __shfl_xor(v, o, 64));
-    return v;
+    const int x = __float_as_int(v);
+    const float u = __int_as_float(__builtin_amdgcn_update_dpp(x, x, CTRL, ROWS, 0xf, false));
+    return MAX ? fmaxf(v, u) : fminf(v, u);
 }
-__device__ inline float wmax(float v)
+
+template <bool MAX>
+__device__ inline float wextreme(float v)
 {
-    for (int o = 32; o > 0; o >>= 1)
-        v = fmaxf(v, __shfl_xor(v, o, 64));
-    return v;
+    v = wextreme_step<MAX, 0x121, 0xf>(v);  // row_ror:1
+    v = wextreme_step<MAX, 0x122, 0xf>(v);  // row_ror:2
+    v = wextreme_step<MAX, 0x124, 0xf>(v);  // row_ror:4
+    v = wextreme_step<MAX, 0x128, 0xf>(v);  // row_ror:8
+    v = wextreme_step<MAX, 0x142, 0xa>(v);  // row_bcast:15
+    v = wextreme_step<MAX, 0x143, 0xc>(v);  // row_bcast:31
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
+__device__ inline float wmin(float v) { return wextreme<false>(v); }
+__device__ inline float wmax(float v) { return wextreme<true>(v); }
+// Inclusive scan over the 64 lanes (all active) in six DPP adds: shifts of
+// 1, 2, 4 and 8 within each 16-lane row, then lane 15 of each row into the
+// next (rows 1 and 3) and lane 31 into rows 2 and 3. Lanes a shift leaves
+// without a source, and rows the row mask excludes, add the old value 0.
+// (Written with lane shuffles, each step is an LDS permute and a wait.)
 __device__ inline uint32_t winclusive(uint32_t v)
 {
-    const int lane = threadIdx.x;
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t u = __shfl_up(v, o, 64);
-        if (lane >= o)
-            v += u;
-    }
-    return v;
+    int x = static_cast<int>(v);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return static_cast<uint32_t>(x);
+}
+
+// the value of lane 63 (a wave-uniform scalar)
+__device__ inline uint32_t wlast(uint32_t v)
+{
+    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
 }
 
 // TreePM short-range weight of the Newtonian pair force:
@@ -744,7 +770,7 @@ __global__ __launch_bounds__(64, FAR ? 1 : BH_WALK_MIN_WAVES) void walk_kernel(
             // 2) children of opened nodes -> traversal stack
             const uint32_t nc = open ? meta_nchild(w.meta) : 0u;
             const uint32_t cinc = winclusive(nc);
-            const int ctot = static_cast<int>(__shfl(cinc, 63, 64));
+            const int ctot = static_cast<int>(wlast(cinc));
             if (sp + ctot > stack_cap) {
                 if (lane == 0)
                     atomicOr(err, kErrStackOverflow);
@@ -762,7 +788,7 @@ __global__ __launch_bounds__(64, FAR ? 1 : BH_WALK_MIN_WAVES) void walk_kernel(
             const int nd = static_cast<int>(__popcll(dm));
             const uint32_t lc = direct ? meta_count(w.meta) : 0u;
             const uint32_t linc = winclusive(lc);
-            const uint32_t ltot = __shfl(linc, 63, 64);
+            const uint32_t ltot = wlast(linc);
             if (direct) {
                 const int slot = static_cast<int>(__popcll(dm & lt));
                 lfFirst[slot] = w.first;
