@@ -99,6 +99,38 @@ def test_topology_bit_exact_hernquist_merger_8388608(oracle):
         assert np.array_equal(t[key].astype(np.int64), ref[key].astype(np.int64)), key
 
 
+@pytest.mark.parametrize("bucket", [1, 16, 64])
+def test_topology_bit_exact_coincident_runs(oracle, bucket):
+    """Runs of exactly coincident particles (equal keys down to level 21)
+    beside a uniform background: 5,000 of them, longer than a topology tile
+    of 4,096 keys and so across a tile boundary; 65, one more than the tiles'
+    halo; 64, the halo exactly; 17, one more than the default bucket. Each run
+    ends as one level-21 leaf holding all of it."""
+    n = 12000
+    say(f"case topology coincident runs n={n} bucket={bucket}")
+    rng = np.random.default_rng(77)
+    pos = rng.random((n, 3))
+    pos[100:5100] = (0.25, 0.5, 0.75)
+    pos[6000:6065] = (0.8, 0.1, 0.3)
+    pos[7000:7064] = (0.6, 0.6, 0.6)
+    pos[8000:8017] = (0.1, 0.9, 0.2)
+    m = np.full(n, 1.0 / n, np.float32)
+    with _solver(n, leaf_capacity=16, bucket_capacity=bucket) as s:
+        s.upload(pos, None, m)
+        s.build_tree()
+        t = s.dump_tree()
+    sk, order = oracle.stable_sort(oracle.keys(pos, 0))
+    assert np.array_equal(t["sorted_keys"], sk)
+    assert np.array_equal(t["sorted_ids"], order)
+    ref = oracle.hip_tree(sk, oracle.leaf_depth(n, 16), bucket)
+    assert len(t["level"]) == len(ref["level"])
+    for key in ("level", "prefix", "start", "count", "first_child", "nchild"):
+        assert np.array_equal(t[key].astype(np.int64), ref[key].astype(np.int64)), key
+    # the 5,000-particle run is one leaf at level 21
+    leaf = (ref["level"] == 21) & (ref["count"] == 5000) & (ref["nchild"] == 0)
+    assert leaf.sum() == 1
+
+
 @pytest.mark.parametrize("key_mode", [0, 1])
 def test_reference_tree_is_truncation(oracle, key_mode):
     n = 30000
