@@ -668,6 +668,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void lf
     __shared__ uint32_t sC[kApplyChunk];
     __shared__ __align__(16) float sB[144];
     __shared__ float sF[12];  // current node: T / (M s^3) (10), 1/s
+    __shared__ __align__(16) float sPF[64][8];  // pair features of up to four tiles
+    __shared__ float sD2[64];                   // and the pairs' |x - c|^2
     const int lane = threadIdx.x, r = lane & 15, q = lane >> 4;
     const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const float ux = geom->unit_f[0], uy = geom->unit_f[1], uz = geom->unit_f[2];
@@ -758,28 +760,46 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void lf
                         sF[10] = nf.inv_s;
                 }
                 __syncthreads();
-                for (int t0 = 0; t0 < nin; t0 += 16) {
-                    const int p = t0 + r;
-                    const bool valid = p < nin;
-                    const int tgt = valid ? sIdx[p] : 0;
-                    const float rx = sT[tgt][0] - ncx, ry = sT[tgt][1] - ncy,
-                                rz = sT[tgt][2] - ncz;
-                    const float d2 = rx * rx + ry * ry + rz * rz;
-                    bf16x4 x0 = xn0, x1 = xn1;
-                    if (q == 3) {
+                // Pair features of up to four tiles at once, one pair per
+                // lane (inside a tile only the 16 lanes of quarter 3 need
+                // them: computing them there left the other 48 idle), then
+                // the tiles in order.
+                for (int t4 = 0; t4 < nin; t4 += 64) {
+                    {
+                        const int p = t4 + lane;
+                        const int tgt = p < nin ? sIdx[p] : 0;
+                        const float rx = sT[tgt][0] - ncx, ry = sT[tgt][1] - ncy,
+                                    rz = sT[tgt][2] - ncz;
+                        const float d2 = rx * rx + ry * ry + rz * rz;
                         float P8[8];
                         pair_feat(sF, sF[10], rx, ry, rz, d2, eps2, P8);
-                        x0 = pack4(P8[0], P8[1], P8[2], P8[3]);
-                        x1 = pack4(P8[4], P8[5], P8[6], P8[7]);
+#pragma unroll
+                        for (int j = 0; j < 8; ++j)
+                            sPF[lane][j] = P8[j];
+                        sD2[lane] = d2;
                     }
-                    const f32x4 c = mlp_mfma(R, sB, q, x0, x1);
-                    if (q == 0 && valid) {
-                        const float mono = w.mass * __builtin_amdgcn_rcpf(d2 + eps2);
-                        sA[tgt][0] += c[0] * mono;
-                        sA[tgt][1] += c[1] * mono;
-                        sA[tgt][2] += c[2] * mono;
+                    __syncthreads();
+                    const int tn = min(64, nin - t4);
+                    for (int t0 = 0; t0 < tn; t0 += 16) {
+                        const int u = t0 + r;
+                        const bool valid = u < tn;
+                        bf16x4 x0 = xn0, x1 = xn1;
+                        if (q == 3) {
+                            const float* P8 = sPF[u];
+                            x0 = pack4(P8[0], P8[1], P8[2], P8[3]);
+                            x1 = pack4(P8[4], P8[5], P8[6], P8[7]);
+                        }
+                        const f32x4 c = mlp_mfma(R, sB, q, x0, x1);
+                        if (q == 0 && valid) {
+                            const int tgt = sIdx[t4 + u];
+                            const float mono = w.mass * __builtin_amdgcn_rcpf(sD2[u] + eps2);
+                            sA[tgt][0] += c[0] * mono;
+                            sA[tgt][1] += c[1] * mono;
+                            sA[tgt][2] += c[2] * mono;
+                        }
+                        ++ntiles;
                     }
-                    ++ntiles;
+                    __syncthreads();  // sPF is rewritten by the next four tiles
                 }
                 npairs += static_cast<unsigned long long>(nin);
                 __syncthreads();  // sIdx is rewritten by the next node
