@@ -663,11 +663,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void lf
     __shared__ float sT[64][3];
     __shared__ float sA[64][3];
     __shared__ int sIdx[64];
-    __shared__ WalkNode sN[kApplyChunk];
-    __shared__ float sO[kApplyChunk][10];
+    // per staged node: position relative to the group (x, y, z), cell size;
+    // mass; particle count; the node part of the input fragments of lane
+    // quarters 0..2 (two halves each); T / (M s^3) (10) and 1/s
+    __shared__ float4 sNG[kApplyChunk];
+    __shared__ float sNM[kApplyChunk];
     __shared__ uint32_t sC[kApplyChunk];
+    __shared__ bf16x4 sXN[kApplyChunk][6];
+    __shared__ float sFN[kApplyChunk][12];
     __shared__ __align__(16) float sB[144];
-    __shared__ float sF[12];  // current node: T / (M s^3) (10), 1/s
     __shared__ __align__(16) float sPF[64][8];  // pair features of up to four tiles
     __shared__ float sD2[64];                   // and the pairs' |x - c|^2
     const int lane = threadIdx.x, r = lane & 15, q = lane >> 4;
@@ -703,25 +707,44 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void lf
             const int kn = min(kApplyChunk, nl - k0);
             __syncthreads();
             if (lane < kn) {
+                // one node per lane: its record, and everything the pairs of
+                // this node share, computed once here rather than by every
+                // lane of the wave per node
                 const int node = static_cast<int>(lists[static_cast<size_t>(g) * list_cap + k0 +
                                                         lane]);
-                sN[lane] = nodes[node];
-                sC[lane] = ncount[node];
-                const float* o = noct + static_cast<size_t>(node) * 10;
+                const WalkNode w = nodes[node];
+                const uint32_t cnt = ncount[node];
+                float o[10];
 #pragma unroll
                 for (int j = 0; j < 10; ++j)
-                    sO[lane][j] = o[j];
+                    o[j] = noct[static_cast<size_t>(node) * 10 + j];
+                const float size = cell_size(wfmax, static_cast<int>(meta_level(w.meta)));
+                sNG[lane] = make_float4(rel_pos(w.cx - rcx, ux, w.comx),
+                                        rel_pos(w.cy - rcy, uy, w.comy),
+                                        rel_pos(w.cz - rcz, uz, w.comz), size);
+                sNM[lane] = w.mass;
+                sC[lane] = cnt;
+                const NodeFeat nf = node_feat(w, o, size, cnt);
+#pragma unroll
+                for (int qq = 0; qq < 3; ++qq) {
+                    sXN[lane][2 * qq] = pack4(nf.f[4 * qq], nf.f[4 * qq + 1], nf.f[4 * qq + 2],
+                                              nf.f[4 * qq + 3]);
+                    sXN[lane][2 * qq + 1] =
+                        pack4(nf.f[12 + 4 * qq], nf.f[13 + 4 * qq], nf.f[14 + 4 * qq],
+                              nf.f[15 + 4 * qq]);
+                }
+#pragma unroll
+                for (int j = 0; j < 10; ++j)
+                    sFN[lane][j] = nf.f[12 + j];
+                sFN[lane][10] = nf.inv_s;
             }
             __syncthreads();
             for (int k = 0; k < kn; ++k) {
                 const uint32_t cnt = sC[k];
                 if (cnt < min_count)
                     continue;  // uniform across the wave
-                const WalkNode w = sN[k];
-                const float size = cell_size(wfmax, static_cast<int>(meta_level(w.meta)));
-                const float ncx = rel_pos(w.cx - rcx, ux, w.comx);
-                const float ncy = rel_pos(w.cy - rcy, uy, w.comy);
-                const float ncz = rel_pos(w.cz - rcz, uz, w.comz);
+                const float4 ng = sNG[k];
+                const float ncx = ng.x, ncy = ng.y, ncz = ng.z, size = ng.w;
                 // compact the targets in the band
                 bool in = false;
                 if (lane < nact) {
@@ -735,31 +758,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void lf
                 if (in)
                     sIdx[__popcll(im & lt)] = lane;
                 __syncthreads();
-                bf16x4 xn0, xn1;
-                {
-                    const NodeFeat nf = node_feat(w, sO[k], size, cnt);
-                    // node part of this lane quarter's input fragments
-                    // (quarter 3 holds the pair part); selects with constant
-                    // indices: a lane-dependent index into nf.f would put it
-                    // in scratch
-                    float v[8];
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        v[j] = q == 0 ? nf.f[j] : q == 1 ? nf.f[4 + j] : nf.f[8 + j];
-                        v[4 + j] = q == 0 ? nf.f[12 + j] : q == 1 ? nf.f[16 + j] : nf.f[20 + j];
-                    }
-                    xn0 = pack4(v[0], v[1], v[2], v[3]);
-                    xn1 = pack4(v[4], v[5], v[6], v[7]);
-                    if (lane < 10)
-                        sF[lane] = lane == 0 ? nf.f[12] : lane == 1 ? nf.f[13]
-                                 : lane == 2 ? nf.f[14] : lane == 3 ? nf.f[15]
-                                 : lane == 4 ? nf.f[16] : lane == 5 ? nf.f[17]
-                                 : lane == 6 ? nf.f[18] : lane == 7 ? nf.f[19]
-                                 : lane == 8 ? nf.f[20] : nf.f[21];
-                    if (lane == 10)
-                        sF[10] = nf.inv_s;
-                }
-                __syncthreads();
+                // node part of this lane quarter's input fragments (quarter 3
+                // carries the pair part, filled per tile below)
+                const int qs = q < 3 ? q : 2;
+                const bf16x4 xn0 = sXN[k][2 * qs], xn1 = sXN[k][2 * qs + 1];
+                const float* sF = sFN[k];
+                const float mass = sNM[k];
                 // Pair features of up to four tiles at once, one pair per
                 // lane (inside a tile only the 16 lanes of quarter 3 need
                 // them: computing them there left the other 48 idle), then
@@ -792,7 +796,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void lf
                         const f32x4 c = mlp_mfma(R, sB, q, x0, x1);
                         if (q == 0 && valid) {
                             const int tgt = sIdx[t4 + u];
-                            const float mono = w.mass * __builtin_amdgcn_rcpf(sD2[u] + eps2);
+                            const float mono = mass * __builtin_amdgcn_rcpf(sD2[u] + eps2);
                             sA[tgt][0] += c[0] * mono;
                             sA[tgt][1] += c[1] * mono;
                             sA[tgt][2] += c[2] * mono;
