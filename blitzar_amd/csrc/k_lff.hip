@@ -637,16 +637,19 @@ __global__ __launch_bounds__(64) void lff_forward_kernel(const float* __restrict
 }
 
 // Per group: for every recorded node, the targets with size / |x - c| >=
-// band are compacted into tiles of 16 pairs (one node, distinct targets), so
-// no MFMA tile carries targets outside the band except the node's last; each
-// pair adds MLP(features) * M / (|x - c|^2 + eps^2) to its target. A tile's
-// pairs have distinct targets and tiles run in order, so the sums are
-// deterministic. Then the acceleration clamp.
+// band form (node, target) pairs; the pairs of consecutive nodes fill tiles
+// of 16 back to back (one MFMA row per pair, so a tile may mix nodes), and
+// only the last tile of a chunk of nodes is partly empty. Each pair adds
+// MLP(features) * M / (|x - c|^2 + eps^2) to its target with an LDS atomic
+// (rows of a tile may share a target); the atomics of one instruction apply
+// in lane order and tiles run in order, so the sums are deterministic. Then
+// the acceleration clamp.
 // Groups are handed out by an atomic ticket (their lists differ widely in
 // length). The recorded nodes of a group are staged 64 at a time: each lane
-// loads one node's record, particle count and third moments into LDS, so the
-// loads of a chunk are in flight together instead of one dependent chain per
-// node.
+// loads one node's record, particle count and third moments and computes
+// what the node's pairs share (node features, packed input fragments,
+// position), so the loads of a chunk are in flight together and the node
+// arithmetic runs once per node.
 constexpr int kApplyChunk = 64;
 
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void lff_apply_kernel(
@@ -662,7 +665,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void lf
 {
     __shared__ float sT[64][3];
     __shared__ float sA[64][3];
-    __shared__ int sIdx[64];
+    __shared__ uint32_t sPair[128];  // pending pairs: node in the chunk << 8 | target
     // per staged node: position relative to the group (x, y, z), cell size;
     // mass; particle count; the node part of the input fragments of lane
     // quarters 0..2 (two halves each); T / (M s^3) (10) and 1/s
@@ -739,75 +742,87 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void lf
                 sFN[lane][10] = nf.inv_s;
             }
             __syncthreads();
+            // The in-band pairs of the chunk's nodes, in node order, fill
+            // 16-pair tiles back to back (a tile's rows may belong to
+            // different nodes); they are evaluated 64 at a time.
+            auto run_pairs = [&](int np) {
+                {
+                    // pair features, one pair per lane
+                    const uint32_t pr = sPair[lane < np ? lane : 0];
+                    const int k = static_cast<int>(pr >> 8), tgt = static_cast<int>(pr & 255u);
+                    const float4 ng = sNG[k];
+                    const float rx = sT[tgt][0] - ng.x, ry = sT[tgt][1] - ng.y,
+                                rz = sT[tgt][2] - ng.z;
+                    const float d2 = rx * rx + ry * ry + rz * rz;
+                    float P8[8];
+                    pair_feat(sFN[k], sFN[k][10], rx, ry, rz, d2, eps2, P8);
+#pragma unroll
+                    for (int j = 0; j < 8; ++j)
+                        sPF[lane][j] = P8[j];
+                    sD2[lane] = d2;
+                }
+                __syncthreads();
+                for (int t0 = 0; t0 < np; t0 += 16) {
+                    const int u = t0 + r;
+                    const bool valid = u < np;
+                    const uint32_t pr = sPair[valid ? u : 0];
+                    const int k = static_cast<int>(pr >> 8), tgt = static_cast<int>(pr & 255u);
+                    // quarters 0..2: the node part of the row's input; 3:
+                    // the pair part
+                    bf16x4 x0, x1;
+                    if (q < 3) {
+                        x0 = sXN[k][2 * q];
+                        x1 = sXN[k][2 * q + 1];
+                    } else {
+                        const float* P8 = sPF[u];
+                        x0 = pack4(P8[0], P8[1], P8[2], P8[3]);
+                        x1 = pack4(P8[4], P8[5], P8[6], P8[7]);
+                    }
+                    const f32x4 c = mlp_mfma(R, sB, q, x0, x1);
+                    if (q == 0 && valid) {
+                        // rows of one tile may share a target: LDS atomics,
+                        // applied in lane order within the instruction
+                        const float mono = sNM[k] * __builtin_amdgcn_rcpf(sD2[u] + eps2);
+                        atomicAdd(&sA[tgt][0], c[0] * mono);
+                        atomicAdd(&sA[tgt][1], c[1] * mono);
+                        atomicAdd(&sA[tgt][2], c[2] * mono);
+                    }
+                    ++ntiles;
+                }
+                __syncthreads();  // sPF and sPair are rewritten next
+            };
+            int pend = 0;  // pairs waiting in sPair (at most 127)
             for (int k = 0; k < kn; ++k) {
-                const uint32_t cnt = sC[k];
-                if (cnt < min_count)
+                if (sC[k] < min_count)
                     continue;  // uniform across the wave
                 const float4 ng = sNG[k];
-                const float ncx = ng.x, ncy = ng.y, ncz = ng.z, size = ng.w;
-                // compact the targets in the band
                 bool in = false;
                 if (lane < nact) {
-                    const float rx = tx - ncx, ry = ty - ncy, rz = tz - ncz;
-                    in = size * size >= band2 * (rx * rx + ry * ry + rz * rz);
+                    const float rx = tx - ng.x, ry = ty - ng.y, rz = tz - ng.z;
+                    in = ng.w * ng.w >= band2 * (rx * rx + ry * ry + rz * rz);
                 }
                 const unsigned long long im = __ballot(in);
                 if (im == 0ull)
                     continue;
-                const int nin = static_cast<int>(__popcll(im));
                 if (in)
-                    sIdx[__popcll(im & lt)] = lane;
+                    sPair[pend + static_cast<int>(__popcll(im & lt))] =
+                        (static_cast<uint32_t>(k) << 8) | static_cast<uint32_t>(lane);
+                pend += static_cast<int>(__popcll(im));
+                npairs += __popcll(im);
                 __syncthreads();
-                // node part of this lane quarter's input fragments (quarter 3
-                // carries the pair part, filled per tile below)
-                const int qs = q < 3 ? q : 2;
-                const bf16x4 xn0 = sXN[k][2 * qs], xn1 = sXN[k][2 * qs + 1];
-                const float* sF = sFN[k];
-                const float mass = sNM[k];
-                // Pair features of up to four tiles at once, one pair per
-                // lane (inside a tile only the 16 lanes of quarter 3 need
-                // them: computing them there left the other 48 idle), then
-                // the tiles in order.
-                for (int t4 = 0; t4 < nin; t4 += 64) {
-                    {
-                        const int p = t4 + lane;
-                        const int tgt = p < nin ? sIdx[p] : 0;
-                        const float rx = sT[tgt][0] - ncx, ry = sT[tgt][1] - ncy,
-                                    rz = sT[tgt][2] - ncz;
-                        const float d2 = rx * rx + ry * ry + rz * rz;
-                        float P8[8];
-                        pair_feat(sF, sF[10], rx, ry, rz, d2, eps2, P8);
-#pragma unroll
-                        for (int j = 0; j < 8; ++j)
-                            sPF[lane][j] = P8[j];
-                        sD2[lane] = d2;
-                    }
+                if (pend >= 64) {
+                    run_pairs(64);
+                    // keep the rest for the next tiles
+                    const uint32_t rest = lane < pend - 64 ? sPair[64 + lane] : 0u;
                     __syncthreads();
-                    const int tn = min(64, nin - t4);
-                    for (int t0 = 0; t0 < tn; t0 += 16) {
-                        const int u = t0 + r;
-                        const bool valid = u < tn;
-                        bf16x4 x0 = xn0, x1 = xn1;
-                        if (q == 3) {
-                            const float* P8 = sPF[u];
-                            x0 = pack4(P8[0], P8[1], P8[2], P8[3]);
-                            x1 = pack4(P8[4], P8[5], P8[6], P8[7]);
-                        }
-                        const f32x4 c = mlp_mfma(R, sB, q, x0, x1);
-                        if (q == 0 && valid) {
-                            const int tgt = sIdx[t4 + u];
-                            const float mono = mass * __builtin_amdgcn_rcpf(sD2[u] + eps2);
-                            sA[tgt][0] += c[0] * mono;
-                            sA[tgt][1] += c[1] * mono;
-                            sA[tgt][2] += c[2] * mono;
-                        }
-                        ++ntiles;
-                    }
-                    __syncthreads();  // sPF is rewritten by the next four tiles
+                    if (lane < pend - 64)
+                        sPair[lane] = rest;
+                    pend -= 64;
+                    __syncthreads();
                 }
-                npairs += static_cast<unsigned long long>(nin);
-                __syncthreads();  // sIdx is rewritten by the next node
             }
+            if (pend > 0)
+                run_pairs(pend);  // before the next chunk restages its nodes
         }
         __syncthreads();
         if (lane < nact) {
