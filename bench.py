@@ -143,7 +143,9 @@ def hbm_fractions(tm, n, walks, args):
       keys       36 B/particle (FP64 positions in, 63-bit key + index out);
       sort      192 B/particle (8 passes x 24 B, the fixed 8-bit-digit
                 convention, whatever the pass count);
-      permute   116 B/particle (+8 B when the relative criterion carries |a|);
+      permute    60 B/particle (positions and masses: the rest of the store
+                 follows on the second stream beside the moments, outside
+                 this timer);
       tree       40 B/particle + 128 B/node (levels, moments, leaf packing
                 and walk groups together);
       integrate 108 B/particle per kick + drift (the closing half kick of a
@@ -152,8 +154,7 @@ def hbm_fractions(tm, n, walks, args):
     ph = {
         "keys": (36.0 * n, tm["keys_ms"] / walks),
         "sort": (192.0 * n, tm["sort_ms"] / walks),
-        "permute": ((124.0 if args.criterion == "relative" else 116.0) * n,
-                    tm["permute_ms"] / walks),
+        "permute": (60.0 * n, tm["permute_ms"] / walks),
         "tree": (40.0 * n + 128.0 * nodes, (tm["tree_ms"] + tm["moments_ms"]) / walks),
         "integrate": (108.0 * n, tm["integrate_ms"] / max(1, args.steps)),
     }

@@ -448,17 +448,19 @@ void queue_build(bh_hip_handle* h, StepEvents* ev)
     rec(ev ? ev->s1 : nullptr, d.stream);
     bh::launch_tie_fix(d);
     // The topology needs only the sorted keys: it runs on the side stream
-    // while the main stream permutes the particle store. The walk groups
-    // follow it there, beside the moments.
+    // while the main stream permutes the positions and masses. The rest of
+    // the store (velocities, ids, |a|) and the walk groups follow the
+    // topology there, beside the moments; the walk waits for both streams.
     (void)hipEventRecord(h->ev_sorted, d.stream);
     (void)hipStreamWaitEvent(h->side, h->ev_sorted, 0);
     bh::launch_tree_build(d, h->leaf_depth, h->p.bucket_capacity, h->side);
     rec(ev ? ev->t1 : nullptr, h->side);
     (void)hipEventRecord(h->ev_topo, h->side);
+    bh::launch_permute_rest(d, h->side);
     if (h->p.key_mode == 0)
         bh::launch_groups(d, h->side);
     (void)hipEventRecord(h->ev_groups, h->side);
-    bh::launch_permute(d);
+    bh::launch_permute_store(d, d.stream);
     rec(ev ? ev->p1 : nullptr, d.stream);
     h->rank_valid = false;
     (void)hipStreamWaitEvent(d.stream, h->ev_topo, 0);

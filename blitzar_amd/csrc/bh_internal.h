@@ -304,7 +304,10 @@ void launch_scan_exclusive(uint32_t* data, size_t max_n, const int* n_dev, size_
                            hipStream_t s);
 size_t scan_block_sums_words(size_t max_n);
 void launch_tie_fix(Device& d);
-void launch_permute(Device& d);
+// sorted order for the store: positions and masses (what the moments read),
+// then velocities, ids, |a| and the block time-step arrays, each on its stream
+void launch_permute_store(Device& d, hipStream_t s);
+void launch_permute_rest(Device& d, hipStream_t s);
 void launch_rank(Device& d);  // rebuild rank from id
 // k_tree.hip
 void launch_tree_build(Device& d, int leaf_depth, int bucket, hipStream_t s);

@@ -193,14 +193,15 @@ __global__ __launch_bounds__(256) void tie_fix_kernel(const unsigned long long* 
     }
 }
 
-__global__ __launch_bounds__(256) void permute_kernel(
+// The store follows the sort in two parts. Positions and masses, which the
+// moments need, go first on the build's main stream; velocities, ids and the
+// relative criterion's |a|, needed only by the walk and the integrator, go
+// on the second stream beside the moments.
+__global__ __launch_bounds__(256) void permute_store_kernel(
     const uint32_t* __restrict__ src, size_t n, const double* __restrict__ px,
-    const double* __restrict__ py, const double* __restrict__ pz, const double* __restrict__ vx,
-    const double* __restrict__ vy, const double* __restrict__ vz, const float* __restrict__ m,
-    const uint32_t* __restrict__ id, const float* __restrict__ am, double* __restrict__ opx,
-    double* __restrict__ opy, double* __restrict__ opz, double* __restrict__ ovx,
-    double* __restrict__ ovy, double* __restrict__ ovz, float* __restrict__ om,
-    uint32_t* __restrict__ oid, float* __restrict__ oam)
+    const double* __restrict__ py, const double* __restrict__ pz, const float* __restrict__ m,
+    double* __restrict__ opx, double* __restrict__ opy, double* __restrict__ opz,
+    float* __restrict__ om)
 {
     const size_t s = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (s >= n)
@@ -209,10 +210,23 @@ __global__ __launch_bounds__(256) void permute_kernel(
     opx[s] = px[j];
     opy[s] = py[j];
     opz[s] = pz[j];
+    om[s] = m[j];
+}
+
+__global__ __launch_bounds__(256) void permute_rest_kernel(
+    const uint32_t* __restrict__ src, size_t n, const double* __restrict__ vx,
+    const double* __restrict__ vy, const double* __restrict__ vz,
+    const uint32_t* __restrict__ id, const float* __restrict__ am, double* __restrict__ ovx,
+    double* __restrict__ ovy, double* __restrict__ ovz, uint32_t* __restrict__ oid,
+    float* __restrict__ oam)
+{
+    const size_t s = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= n)
+        return;
+    const uint32_t j = src[s];
     ovx[s] = vx[j];
     ovy[s] = vy[j];
     ovz[s] = vz[j];
-    om[s] = m[j];
     oid[s] = id[j];
     if (am)
         oam[s] = am[j];
@@ -339,25 +353,34 @@ void launch_tie_fix(Device& d)
                                                 d.vals, d.id, d.n);
 }
 
-void launch_permute(Device& d)
+void launch_permute_store(Device& d, hipStream_t st)
 {
     if (d.n == 0)
         return;
     const unsigned blocks = static_cast<unsigned>((d.n + 255) / 256);
-    permute_kernel<<<blocks, 256, 0, d.stream>>>(
-        d.vals, d.n, d.pos[0], d.pos[1], d.pos[2], d.vel[0], d.vel[1], d.vel[2], d.mass, d.id,
-        d.amag_live ? d.amag : nullptr, d.pos_b[0], d.pos_b[1], d.pos_b[2], d.vel_b[0], d.vel_b[1],
-        d.vel_b[2], d.mass_b, d.id_b, d.amag_b);
-    for (int a = 0; a < 3; ++a) {
+    permute_store_kernel<<<blocks, 256, 0, st>>>(d.vals, d.n, d.pos[0], d.pos[1], d.pos[2],
+                                                 d.mass, d.pos_b[0], d.pos_b[1], d.pos_b[2],
+                                                 d.mass_b);
+    for (int a = 0; a < 3; ++a)
         std::swap(d.pos[a], d.pos_b[a]);
-        std::swap(d.vel[a], d.vel_b[a]);
-    }
     std::swap(d.mass, d.mass_b);
+}
+
+void launch_permute_rest(Device& d, hipStream_t st)
+{
+    if (d.n == 0)
+        return;
+    const unsigned blocks = static_cast<unsigned>((d.n + 255) / 256);
+    permute_rest_kernel<<<blocks, 256, 0, st>>>(
+        d.vals, d.n, d.vel[0], d.vel[1], d.vel[2], d.id, d.amag_live ? d.amag : nullptr,
+        d.vel_b[0], d.vel_b[1], d.vel_b[2], d.id_b, d.amag_b);
+    for (int a = 0; a < 3; ++a)
+        std::swap(d.vel[a], d.vel_b[a]);
     std::swap(d.id, d.id_b);
     if (d.amag_live)
         std::swap(d.amag, d.amag_b);
     if (d.alevel) {
-        permute_adaptive_kernel<<<blocks, 256, 0, d.stream>>>(
+        permute_adaptive_kernel<<<blocks, 256, 0, st>>>(
             d.vals, d.n, d.aacc[0], d.aacc[1], d.aacc[2], d.alevel, d.atick, d.aacc_b[0],
             d.aacc_b[1], d.aacc_b[2], d.alevel_b, d.atick_b);
         for (int a = 0; a < 3; ++a)
