@@ -506,34 +506,41 @@ __global__ __launch_bounds__(64, FAR ? 1 : BH_WALK_MIN_WAVES) void walk_kernel(
         const int g = ngroups - 1 - ticket;
 
         const uint4 gr = groups[g];
-        const int nact = static_cast<int>(gr.y);
+        int nact = static_cast<int>(gr.y);
+        // Targets of this walk: the group's particles, or under block time
+        // steps those due a force now. The box, the relative tolerance and
+        // the stored forces cover these only (the corrector reads no other
+        // particle's force, and each particle is due again before its next
+        // kick), so a group with a few due particles walks with their box.
+        // The due particles are compacted to the first lanes (their offsets
+        // pass through lfFirst, unused until the traversal), so that a group
+        // with few of them splits its lists like a small group.
+        if (P.alevel) {
+            bool due = false;
+            if (lane < nact) {
+                const uint32_t lv = min(P.alevel[gr.x + static_cast<uint32_t>(lane)],
+                                        static_cast<uint32_t>(P.alevels));
+                due = (P.atick & ((1u << (static_cast<uint32_t>(P.alevels) - lv)) - 1u)) == 0u;
+            }
+            const unsigned long long dm = __ballot(due);
+            if (dm == 0ull)
+                continue;
+            if (due)
+                lfFirst[__popcll(dm & lt)] = lane;
+            nact = static_cast<int>(__popcll(dm));
+            __syncthreads();
+        }
         // groups of <= 32 (<= 16) targets: 2 (4) lanes per target, each
         // evaluating every 2nd (4th) list entry
         const int dj = nact <= 16 ? 4 : (nact <= 32 ? 2 : 1);
         const int tw = 64 / dj;  // lanes per list share
         const int tl = lane & (tw - 1);
         const int j0 = lane / tw;
-        const int s = static_cast<int>(gr.x) + tl;
         const bool active = tl < nact;
+        const int s = static_cast<int>(gr.x) + (P.alevel ? (active ? lfFirst[tl] : 0) : tl);
+        const bool target = active;
         const int gnode = static_cast<int>(gr.z);
         const int rcx = nodes[gnode].cx, rcy = nodes[gnode].cy, rcz = nodes[gnode].cz;
-
-        // Targets of this walk: the group's particles, or under block time
-        // steps those due a force now. The box, the relative tolerance and
-        // the stored forces cover these only (the corrector reads no other
-        // particle's force, and each particle is due again before its next
-        // kick), so a group with a few due particles walks with their box.
-        bool target = active;
-        if (P.alevel) {
-            bool due = false;
-            if (active) {
-                const uint32_t lv = min(P.alevel[s], static_cast<uint32_t>(P.alevels));
-                due = (P.atick & ((1u << (static_cast<uint32_t>(P.alevels) - lv)) - 1u)) == 0u;
-            }
-            if (!__any(due))
-                continue;
-            target = due;
-        }
 
         float tx = 0.0f, ty = 0.0f, tz = 0.0f;
         if (active) {
