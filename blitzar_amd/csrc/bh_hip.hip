@@ -289,7 +289,9 @@ bool alloc_all(bh_hip_handle* h, uint64_t n)
         return false;
     if (!alloc_nodes(h, 2 * n + 4096))
         return false;
-    const size_t groups = (n + 63) / 64;
+    // resident walk waves: one per group of the smallest cap (16, see
+    // group_max) up to the residency limit
+    const size_t groups = (n + 15) / 16;
     d.stack_waves = std::max<size_t>(
         1, std::min(groups, static_cast<size_t>(h->cu_count) * static_cast<size_t>(h->p.waves_per_cu)));
     d.stack_cap = std::min(kStackCap, d.node_cap + 64);
@@ -436,6 +438,30 @@ void rec(hipEvent_t e, hipStream_t s)
         (void)hipEventRecord(e, s);
 }
 
+// Walk-group size. A walk keeps about cu_count * 24 waves resident and
+// hands each group to one wave, so with fewer groups than that it lasts as
+// long as its heaviest group. Small workloads therefore cut smaller groups:
+// the cap halves from 64 while n < 2 * cap * cu_count * 24, down to 16, and
+// the lists of a small group split over 2 or 4 lanes per target. The cap
+// depends on n and the GPU only, never on the requested residency, so
+// schedules do not change results. It holds for the COM and relative
+// criteria without the group far field. The bounds and extent criteria keep
+// 64: with groups of 16 the bounds walk on the reference's 2,000-particle
+// cloud (theta 0.5) erred more than the reference's own per-target walk,
+// which groups of 64 do not. The far field expands about the group centre
+// and is built for groups of 64.
+uint32_t group_max(const bh_hip_handle* h)
+{
+    if (h->p.criterion == BH_HIP_CRITERION_BOUNDS || h->p.criterion == BH_HIP_CRITERION_EXTENT ||
+        h->p.far_alpha > 0.0f)
+        return bh::kGroupMax;
+    const size_t waves = static_cast<size_t>(h->cu_count) * static_cast<size_t>(kDefaultWavesPerCu);
+    uint32_t g = bh::kGroupMax;
+    while (g > 16 && h->d.n < 2 * static_cast<size_t>(g) * waves)
+        g /= 2;
+    return g;
+}
+
 // Queue one tree build for the current positions.
 void queue_build(bh_hip_handle* h, StepEvents* ev)
 {
@@ -458,7 +484,7 @@ void queue_build(bh_hip_handle* h, StepEvents* ev)
     (void)hipEventRecord(h->ev_topo, h->side);
     bh::launch_permute_rest(d, h->side);
     if (h->p.key_mode == 0)
-        bh::launch_groups(d, h->side);
+        bh::launch_groups(d, group_max(h), h->side);
     (void)hipEventRecord(h->ev_groups, h->side);
     bh::launch_permute_store(d, d.stream);
     rec(ev ? ev->p1 : nullptr, d.stream);

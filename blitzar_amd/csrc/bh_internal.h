@@ -317,9 +317,9 @@ void launch_tree_build(Device& d, int leaf_depth, int bucket, hipStream_t s);
 // cell covers the node's particles.
 void launch_tree_moments(Device& d, bool refit = false);
 // Walk groups: a partition of the sorted particles into spatially compact
-// runs of at most kGroupMax (consecutive small children of one node packed
-// together; see node_groups in k_tree.hip).
-void launch_groups(Device& d, hipStream_t s);
+// runs of at most gmax <= kGroupMax particles (consecutive small children of
+// one node packed together; see node_groups in k_tree.hip).
+void launch_groups(Device& d, uint32_t gmax, hipStream_t s);
 // k_walk.hip
 // stats (optional, device): [0] particle-particle, [1] particle-cell interactions,
 // [3] node-group far-field interactions, [4] far-field target evaluations

@@ -1215,7 +1215,8 @@ __global__ __launch_bounds__(256) void tree_moments_kernel(
 //     group whose targets surround a dense sibling would open all of it.
 // Every particle lands in exactly one group; a group's particles are
 // contiguous in sorted order and lie inside node i's cell, whose lattice
-// centre is the group's reference point. G = kGroupMax.
+// centre is the group's reference point. G = gmax (at most kGroupMax, one
+// target per lane of the walk's wavefront).
 __device__ inline bool octant_box_holds(uint32_t ax, uint32_t ay, uint32_t az, uint32_t o)
 {
     // ax/ay/az: bit v set when the box covers value v of that axis' octant bit
@@ -1229,10 +1230,10 @@ __device__ inline uint32_t node_groups(int i, const uint64_t* __restrict__ npref
                                        const uint32_t* __restrict__ nchild,
                                        const uint32_t* __restrict__ nfirst,
                                        const uint32_t* __restrict__ nparent, uint32_t base,
-                                       uint4* __restrict__ groups)
+                                       uint4* __restrict__ groups, uint32_t gmax)
 {
     const uint32_t c = ncount[i];
-    if (c <= kGroupMax) {
+    if (c <= gmax) {
         if (nparent[i] != 0xffffffffu)
             return 0;
         if (EMIT)
@@ -1241,11 +1242,11 @@ __device__ inline uint32_t node_groups(int i, const uint64_t* __restrict__ npref
     }
     const uint32_t nc = nchild[i];
     if (nc == 0) {
-        const uint32_t k = (c + kGroupMax - 1) / kGroupMax;
+        const uint32_t k = (c + gmax - 1) / gmax;
         if (EMIT)
             for (uint32_t j = 0; j < k; ++j) {
-                const uint32_t off = j * kGroupMax;
-                const uint32_t cnt = c - off < kGroupMax ? c - off : kGroupMax;
+                const uint32_t off = j * gmax;
+                const uint32_t cnt = c - off < gmax ? c - off : gmax;
                 groups[base + j] = make_uint4(nstart[i] + off, cnt, static_cast<uint32_t>(i), 0u);
             }
         return k;
@@ -1253,20 +1254,20 @@ __device__ inline uint32_t node_groups(int i, const uint64_t* __restrict__ npref
     const uint32_t f = nfirst[i];
     uint32_t dense = 0;  // octants of the children holding > G particles
     for (uint32_t j = f; j < f + nc; ++j)
-        if (ncount[j] > kGroupMax)
+        if (ncount[j] > gmax)
             dense |= 1u << static_cast<uint32_t>(nprefix[j] & 7u);
     uint32_t ng = 0, run_start = 0, run_count = 0;
     uint32_t bx = 0, by = 0, bz = 0;  // the run's octant box
     for (uint32_t j = f; j < f + nc; ++j) {
         const uint32_t cc = ncount[j];
-        const bool small = cc <= kGroupMax;
+        const bool small = cc <= gmax;
         const uint32_t o = static_cast<uint32_t>(nprefix[j] & 7u);
         const uint32_t nx = bx | (1u << ((o >> 2) & 1u)), ny = by | (1u << ((o >> 1) & 1u)),
                        nz = bz | (1u << (o & 1u));
         bool covers_dense = false;
         for (uint32_t q = 0; q < 8; ++q)
             covers_dense = covers_dense || (((dense >> q) & 1u) && octant_box_holds(nx, ny, nz, q));
-        if (run_count > 0 && (!small || run_count + cc > kGroupMax || covers_dense)) {
+        if (run_count > 0 && (!small || run_count + cc > gmax || covers_dense)) {
             if (EMIT)
                 groups[base + ng] = make_uint4(run_start, run_count, static_cast<uint32_t>(i), 0u);
             ++ng;
@@ -1294,11 +1295,12 @@ __global__ __launch_bounds__(256) void group_count_kernel(
     const uint64_t* __restrict__ nprefix, const uint32_t* __restrict__ nstart,
     const uint32_t* __restrict__ ncount, const uint32_t* __restrict__ nchild,
     const uint32_t* __restrict__ nfirst, const uint32_t* __restrict__ nparent,
-    const int* __restrict__ info, uint32_t* __restrict__ gcnt)
+    const int* __restrict__ info, uint32_t* __restrict__ gcnt, uint32_t gmax)
 {
     const int total = info[kInfoTotal];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x)
-        gcnt[i] = node_groups<false>(i, nprefix, nstart, ncount, nchild, nfirst, nparent, 0, nullptr);
+        gcnt[i] = node_groups<false>(i, nprefix, nstart, ncount, nchild, nfirst, nparent, 0, nullptr,
+                                     gmax);
 }
 
 __global__ __launch_bounds__(256) void group_emit_kernel(
@@ -1306,11 +1308,12 @@ __global__ __launch_bounds__(256) void group_emit_kernel(
     const uint32_t* __restrict__ ncount, const uint32_t* __restrict__ nchild,
     const uint32_t* __restrict__ nfirst, const uint32_t* __restrict__ nparent,
     const uint32_t* __restrict__ goff,
-    const int* __restrict__ info, uint4* __restrict__ groups)
+    const int* __restrict__ info, uint4* __restrict__ groups, uint32_t gmax)
 {
     const int total = info[kInfoTotal];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x)
-        node_groups<true>(i, nprefix, nstart, ncount, nchild, nfirst, nparent, goff[i], groups);
+        node_groups<true>(i, nprefix, nstart, ncount, nchild, nfirst, nparent, goff[i], groups,
+                          gmax);
 }
 
 inline unsigned level_grid(size_t bound)
@@ -1379,15 +1382,15 @@ void launch_tree_build(Device& d, int leaf_depth, int bucket, hipStream_t s)
                        s>>>(d.nstart, d.ncount, d.nchild, d.level_info, n, d.sleaf);
 }
 
-void launch_groups(Device& d, hipStream_t s)
+void launch_groups(Device& d, uint32_t gmax, hipStream_t s)
 {
     const unsigned g = level_grid(d.node_cap);
     group_count_kernel<<<g, 256, 0, s>>>(d.nprefix, d.nstart, d.ncount, d.nchild, d.nfirst,
-                                                d.nparent, d.level_info, d.gscan);
+                                                d.nparent, d.level_info, d.gscan, gmax);
     launch_scan_exclusive(d.gscan, d.node_cap, d.level_info + kInfoTotal, 0, d.gscan_blocks,
                           nullptr, d.level_info + kInfoGroups, d.err, s);
     group_emit_kernel<<<g, 256, 0, s>>>(d.nprefix, d.nstart, d.ncount, d.nchild, d.nfirst,
-                                               d.nparent, d.gscan, d.level_info, d.groups);
+                                               d.nparent, d.gscan, d.level_info, d.groups, gmax);
 }
 
 void launch_tree_moments(Device& d, bool refit)
